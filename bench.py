@@ -1,0 +1,233 @@
+"""Benchmark of the fused Welch path on one MI355X.
+
+Workload (headline): welch over x = 64 channels x 2^20 float32 samples, each
+channel the sum of 3 unit sinusoids (freqs uniform in [0, fs/2), random phase)
+plus N(0,1) noise, fs = 48000, nperseg = 1024, noverlap = 512, hann,
+detrend='constant', density scaling, mean average -> 513 bins per channel.
+Inputs are generated on the device from a seed (synthetic) and are resident
+in HBM before the timed region; three input buffers are rotated so the 256 MB
+last-level cache cannot serve the 256 MiB input.
+
+One step = one ``mlx_signal_amd.welch`` call on a device buffer.  Prints ONE
+JSON line (value = samples/s over the whole job).  Secondary forms (csd,
+coherence at the same shape, stft / spectrogram 16 x 2^20) are reported under
+"secondary".  The CPU baseline is the reference's dispatch='scipy' route
+(scipy.signal.welch on the same float32 data, one process) timed on a bounded
+sample on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-secondary]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT, ROOT / "mlx-signal_amd"):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+HBM_PEAK_GBS = 8000.0
+FS = 48000.0
+C, NS, NPERSEG, NOVERLAP = 64, 1 << 20, 1024, 512
+METRIC = "Welch samples/s + HBM GB/s (64ch×2^20, nperseg=1024); serving p50/p99 latency"
+
+
+def welch_bytes(B, n, bins):
+    """Algorithmic bytes: input read once + output written once."""
+    return 4 * B * n + 4 * B * bins
+
+
+def make_inputs(torch, rows, n, seed, dev, with_y=True):
+    """x = 3 sinusoids + N(0,1); y = FIR32(x) + 0.1 N(0,1) (causal, taps N(0,1)/sqrt(32))."""
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    t = torch.arange(n, device=dev, dtype=torch.float64)
+    x = torch.randn(rows, n, device=dev, generator=g, dtype=torch.float32)
+    f = torch.rand(rows, 3, device=dev, generator=g, dtype=torch.float64) * (FS / 2)
+    ph = torch.rand(rows, 3, device=dev, generator=g, dtype=torch.float64) * (2 * np.pi)
+    for k in range(3):
+        x += torch.sin(2 * np.pi * f[:, k : k + 1] / FS * t[None, :] + ph[:, k : k + 1]).float()
+    if not with_y:
+        return x, None
+    taps = torch.randn(32, device=dev, generator=g, dtype=torch.float32) / np.sqrt(32.0)
+    xp = torch.nn.functional.pad(x[:, None, :], (31, 0))
+    y = torch.nn.functional.conv1d(xp, taps.flip(0)[None, None, :])[:, 0, :]
+    y = y + 0.1 * torch.randn(rows, n, device=dev, generator=g, dtype=torch.float32)
+    return x.contiguous(), y.contiguous()
+
+
+def time_steps(torch, fn, bufs, steps, warmup):
+    """Wall clock over K steps (synchronized both sides) plus per-step HIP event
+    durations on the current stream (the stream the kernels launch on)."""
+    for i in range(warmup):
+        fn(bufs[i % len(bufs)])
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        evs[i][0].record()
+        fn(bufs[i % len(bufs)])
+        evs[i][1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = [a.elapsed_time(b) for a, b in evs]
+    return wall / steps * 1e3, statistics.median(ev_ms), ev_ms
+
+
+def traffic_from_profile():
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    p = ROOT / "profiles" / "welch_64x2p20_traffic.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            return None
+    return None
+
+
+def cpu_baseline():
+    """scipy.signal.welch (the reference's dispatch='scipy' route, restated in
+    oracle.scipy_route) on 8 channels x 2^20 float32, median of 3."""
+    import oracle
+
+    rng = np.random.default_rng(0)
+    rows = 8
+    xs = rng.standard_normal((rows, NS)).astype(np.float32)
+    kw = dict(fs=FS, window="hann", nperseg=NPERSEG, noverlap=NOVERLAP)
+    oracle.scipy_route("welch", xs, **kw)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.scipy_route("welch", xs, **kw)
+        ts.append(time.perf_counter() - t0)
+    t = statistics.median(ts)
+    return {
+        "value": rows * NS / t,
+        "unit": "samples/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"scipy.signal.welch float32 {rows}ch x 2^20, nperseg=1024, noverlap=512 "
+                  f"(reference dispatch='scipy' route), median of 3: {t * 1e3:.1f} ms; "
+                  f"host {os.cpu_count()} logical CPUs, 1 used",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by channel with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import mlx_signal_amd as msig
+    import oracle
+    from mlx_signal_amd import _hip
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    _hip.load()
+    msig.set_config(dispatch="gpu")
+
+    bins = NPERSEG // 2 + 1
+    xs = [make_inputs(torch, C, NS, seed, dev, with_y=False)[0] for seed in range(3)]
+    torch.cuda.synchronize()
+
+    def step(x):
+        return msig.welch(x, fs=FS, window="hann", nperseg=NPERSEG, noverlap=NOVERLAP)
+
+    ms_wall, ms_ev, ev_all = time_steps(torch, step, xs, args.steps, args.warmup)
+    B = welch_bytes(C, NS, bins)
+    achieved = B / (ms_ev * 1e-3) / 1e9
+
+    # accuracy gate on the measured data: 2 channels vs the FP64 oracle
+    _, p = step(xs[0])
+    xr = xs[0][:2].cpu().numpy()
+    ref = oracle.welch(xr, fs=FS, window="hann", nperseg=NPERSEG, noverlap=NOVERLAP)[1]
+    err = float(np.max(np.abs(p[:2].cpu().numpy() - ref)) / np.max(np.abs(ref)))
+
+    secondary = {}
+    if not args.no_secondary:
+        x2, y2 = make_inputs(torch, C, NS, 11, dev)
+        x3, y3 = make_inputs(torch, C, NS, 12, dev)
+        pairs = [(x2, y2), (x3, y3)]
+        kw = dict(fs=FS, window="hann", nperseg=NPERSEG, noverlap=NOVERLAP)
+        for name, fn, nbytes in [
+            ("csd", lambda b: msig.csd(b[0], b[1], **kw), 8 * C * NS + 8 * C * bins),
+            ("coherence", lambda b: msig.coherence(b[0], b[1], **kw), 8 * C * NS + 4 * C * bins),
+        ]:
+            w, e, _ = time_steps(torch, fn, pairs, max(5, args.steps // 2), 2)
+            secondary[name] = {"ms_event": e, "ms_wall": w, "GBps": nbytes / (e * 1e-3) / 1e9,
+                               "pairs_per_s": C * NS / (e * 1e-3)}
+        del x2, y2, x3, y3, pairs
+        xs16 = [x[:16] for x in xs]
+        nseg_stft = (NS + 1024 - 1024) // 512 + 1
+        nseg_sp = (NS - 1024) // 512 + 1
+        for name, fn, nbytes in [
+            ("stft_16x2^20", lambda b: msig.stft(b, nperseg=1024),
+             4 * 16 * NS + 8 * 16 * nseg_stft * bins),
+            ("spectrogram_16x2^20", lambda b: msig.spectrogram(b, nperseg=1024, noverlap=512),
+             4 * 16 * NS + 4 * 16 * nseg_sp * bins),
+        ]:
+            w, e, _ = time_steps(torch, fn, xs16, max(5, args.steps // 2), 2)
+            secondary[name] = {"ms_event": e, "ms_wall": w, "GBps": nbytes / (e * 1e-3) / 1e9,
+                               "samples_per_s": 16 * NS / (e * 1e-3)}
+        # end-to-end: NumPy in -> tensor -> host out (the reference's "e2e" column)
+        xh = xs[0].cpu().numpy()
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            _, pe = msig.welch(xh, fs=FS, nperseg=NPERSEG, noverlap=NOVERLAP)
+            pe.cpu()
+            ts.append(time.perf_counter() - t0)
+        secondary["welch_e2e_numpy_ms"] = statistics.median(ts) * 1e3
+
+    out = {
+        "metric": METRIC,
+        "value": C * NS / (ms_ev * 1e-3),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_wall,
+        "ms_per_step_event": ms_ev,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device-generated: 3 sinusoids + N(0,1) noise, seeded)",
+        "config": {"workload": "welch 64ch x 2^20 f32, fs=48000, nperseg=1024, noverlap=512, "
+                               "hann, detrend=constant, density, mean -> 513 bins",
+                   "channels": C, "samples_per_channel": NS, "nperseg": NPERSEG,
+                   "noverlap": NOVERLAP, "parallelism": "single GPU"},
+        "hbm_GBps": achieved,
+        "max_err_vs_fp64_peak": err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile()},
+        "secondary": secondary,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * msig.h - C ABI of the MI355X fused STFT/Welch library (libmsig.so).
+ *
+ * Every entry point takes plain device pointers, element counts and a
+ * hipStream_t passed as void*; nothing throws across the ABI and no torch
+ * type appears.  Calls are asynchronous on the given stream.  The caller owns
+ * x, y, the window, all outputs and the workspace; the library owns only its
+ * device-resident twiddle tables (one per device and FFT size, built on first
+ * use and kept until the process ends).
+ *
+ * Return value: 0 on success, or a negative errno-style status:
+ *   MSIG_EINVAL  (-22) bad descriptor / pointer / size
+ *   MSIG_ENOTSUP (-95) the fused path does not cover this shape (the Python
+ *                      planner then takes its composed path, as the
+ *                      reference does when `eligible` is False)
+ *   MSIG_EIO     (-5)  a HIP runtime error; text via msig_last_error()
+ *
+ * Layouts (all row-major, rows contiguous along the analysed axis):
+ *   real input x, y      float32 [rows][ld]   (the first n samples are used)
+ *   complex input        float32 pairs [rows][ld][2]
+ *   mean outputs         [rows][bins]   bins = nfft/2+1 (onesided) or nfft
+ *   per-frame outputs    [rows][nseg][bins]
+ *   complex outputs      interleaved (re, im) float32 pairs (= complex64)
+ */
+#ifndef MSIG_H
+#define MSIG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSIG_ABI_VERSION 1
+
+#define MSIG_EINVAL (-22)
+#define MSIG_ENOTSUP (-95)
+#define MSIG_EIO (-5)
+
+/* Smallest and largest FFT sizes (powers of two) of the fused kernels. */
+#define MSIG_MIN_NFFT 16
+#define MSIG_MAX_NFFT 4096
+
+/* job kinds */
+enum {
+  MSIG_PSD_MEAN = 0,      /* mean |X|^2 over segments            -> out0 f32     */
+  MSIG_CSD_MEAN = 1,      /* mean conj(X) Y                      -> out0 c64     */
+  MSIG_COH_TRIO_MEAN = 2, /* mean |X|^2, |Y|^2, conj(X) Y        -> out0,1 f32, out2 c64 */
+  MSIG_COHERENCE = 3,     /* |Pxy|^2 / (Pxx Pyy)                 -> out0 f32     */
+  MSIG_FRAMES_COMPLEX = 4,/* per-frame X                         -> out0 c64     */
+  MSIG_FRAMES_POWER = 5,  /* per-frame |X|^2                     -> out0 f32     */
+  MSIG_CSD_FRAMES = 6,    /* per-frame conj(X) Y                 -> out0 c64     */
+  MSIG_COH_FRAMES = 7     /* per-frame |X|^2, |Y|^2, conj(X) Y   -> out0,1 f32, out2 c64 */
+};
+
+/* job flags */
+#define MSIG_F_COMPLEX_INPUT 0x1 /* x (and y) are complex64; output is two-sided  */
+#define MSIG_F_ONESIDED 0x2      /* real input: emit bins 0..nfft/2 only           */
+#define MSIG_F_APPLY_FACTOR 0x4  /* multiply onesided bins 1..nfft/2-1 by 2        */
+#define MSIG_F_DETREND_CONST 0x8 /* subtract each segment's mean before windowing  */
+
+typedef struct msig_job {
+  int32_t kind;    /* MSIG_* kind above                                         */
+  int32_t flags;   /* MSIG_F_* bits                                             */
+  int64_t rows;    /* independent rows (channels)                               */
+  int64_t n;       /* samples per row the segments are cut from                 */
+  int64_t ld_x;    /* row stride of x in elements (>= n)                        */
+  int64_t ld_y;    /* row stride of y in elements (>= n; ignored without y)     */
+  int32_t nperseg; /* segment length (window length), 1 <= nperseg <= nfft      */
+  int32_t nstep;   /* hop between segment starts = nperseg - noverlap, >= 1     */
+  int32_t nfft;    /* FFT length, power of two in [MSIG_MIN_NFFT, MSIG_MAX_NFFT] */
+  int32_t nseg;    /* (n - nperseg) / nstep + 1, checked                        */
+  double scale;    /* multiplies every mean/power output bin; complex per-frame
+                      outputs are multiplied by sqrt(scale)                     */
+} msig_job;
+
+/* ABI version the library was built with (MSIG_ABI_VERSION). */
+int msig_abi_version(void);
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* msig_last_error(void);
+
+/* Number of visible HIP devices (0 when no GPU / no driver), or < 0 on error. */
+int msig_device_count(void);
+
+/* Make `device` current for the calling thread and build its resident
+ * tables for every fused FFT size.  Optional: the entry points build what
+ * they need on first use. */
+int msig_init(int device);
+
+/* 1 if the fused kernels cover this shape, else 0.
+ * Counterpart of `eligible` (reference: src/mlx_signal_processing/_stft_metal.py:150-160),
+ * widened to nfft >= nperseg (zero padding), two-sided and complex input. */
+int msig_eligible(int32_t nperseg, int32_t nfft, int32_t onesided, int32_t detrend_const,
+                  int32_t complex_input);
+
+/* Workspace bytes the mean kinds need (0 for per-frame kinds). */
+int msig_workspace_bytes(const msig_job* job, size_t* bytes);
+
+/* Generic entry points (all kinds).  y may be NULL for single-signal kinds;
+ * win is float32[nperseg] on the device.  The mean kinds write
+ * scale * mean_over_segments(...) (times the onesided factor when
+ * MSIG_F_APPLY_FACTOR is set); MSIG_COHERENCE ignores scale. */
+int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const float* win,
+                       void* out0, void* out1, void* out2, void* workspace, size_t ws_bytes,
+                       void* stream);
+int msig_spectral_frames(const msig_job* job, const void* x, const void* y, const float* win,
+                         void* out0, void* out1, void* out2, void* stream);
+
+/* ---- operator-level entry points: one per function of the reference's fused
+ * module.  Real float32 input, rows contiguous (ld = n), onesided, nfft ==
+ * nperseg, `win` carries any scale (as in the reference), detrend is 0 (none)
+ * or 1 ('constant'); nseg = (n - nperseg) / hop + 1. ---- */
+
+/* rfft_frames (reference: src/mlx_signal_processing/_stft_metal.py:192-220):
+ * power == 0 -> out complex64 [rows][nseg][nperseg/2+1];
+ * power != 0 -> out float32  [rows][nseg][nperseg/2+1]. */
+int msig_rfft_frames(const float* x, int64_t rows, int64_t n, const float* win, int32_t nperseg,
+                     int32_t hop, int32_t detrend, int32_t power, void* out, void* stream);
+
+/* rfft_power_mean (reference: _stft_metal.py:530-557): out float32 [rows][nperseg/2+1]. */
+int msig_rfft_power_mean(const float* x, int64_t rows, int64_t n, const float* win,
+                         int32_t nperseg, int32_t hop, int32_t detrend, float* out,
+                         void* workspace, size_t ws_bytes, void* stream);
+
+/* csd_frames (reference: _stft_metal.py:411-413): out complex64 [rows][nseg][bins]. */
+int msig_csd_frames(const float* x, const float* y, int64_t rows, int64_t n, const float* win,
+                    int32_t nperseg, int32_t hop, int32_t detrend, void* out, void* stream);
+
+/* coherence_frames (reference: _stft_metal.py:416-418): pxx, pyy f32, pxy c64 per frame. */
+int msig_coherence_frames(const float* x, const float* y, int64_t rows, int64_t n,
+                          const float* win, int32_t nperseg, int32_t hop, int32_t detrend,
+                          float* pxx, float* pyy, void* pxy, void* stream);
+
+/* csd_mean (reference: _stft_metal.py:739-741): out complex64 [rows][bins]. */
+int msig_csd_mean(const float* x, const float* y, int64_t rows, int64_t n, const float* win,
+                  int32_t nperseg, int32_t hop, int32_t detrend, void* out, void* workspace,
+                  size_t ws_bytes, void* stream);
+
+/* coherence_mean (reference: _stft_metal.py:744-746): mean pxx, pyy f32 and pxy c64. */
+int msig_coherence_mean(const float* x, const float* y, int64_t rows, int64_t n,
+                        const float* win, int32_t nperseg, int32_t hop, int32_t detrend,
+                        float* pxx, float* pyy, void* pxy, void* workspace, size_t ws_bytes,
+                        void* stream);
+
+/* Workspace bytes for the operator-level mean entry points
+ * (kind = MSIG_PSD_MEAN, MSIG_CSD_MEAN or MSIG_COH_TRIO_MEAN). */
+int msig_mean_workspace_bytes(int32_t kind, int64_t rows, int64_t n, int32_t nperseg, int32_t hop,
+                              size_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSIG_H */

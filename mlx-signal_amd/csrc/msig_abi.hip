@@ -1,0 +1,437 @@
+// C ABI of libmsig.so: validation, the resident twiddle tables, the FP64
+// segment-reduction/finalize kernel, and dispatch to the per-size kernels.
+//
+// Reference interfaces replaced (src/mlx_signal_processing/_stft_metal.py):
+// eligible :150-160, rfft_frames :192-220, csd_frames / coherence_frames
+// :378-419, rfft_power_mean :530-557, csd_mean / coherence_mean :703-746,
+// _group_params / _reduce_groups :429-455.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "msig.h"
+#include "msig_launch.hpp"
+
+namespace msig {
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+static int hip_fail(hipError_t e, const char* what) {
+  return fail(MSIG_EIO, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+// ---------------------------------------------------------------- twiddles
+// exp(-2 pi i e / N), e < N, computed in FP64 and rounded once to float.
+static std::mutex g_tw_mu;
+static std::map<std::pair<int, int>, cf*> g_tw;  // (device, N) -> device table
+
+static int twiddles(int N, const cf** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto it = g_tw.find({dev, N});
+  if (it != g_tw.end()) {
+    *out = it->second;
+    return 0;
+  }
+  std::vector<cf> h(N);
+  for (int i = 0; i < N; ++i) {
+    const double ang = -2.0 * M_PI * (double)i / (double)N;
+    h[i] = cf{(float)std::cos(ang), (float)std::sin(ang)};
+  }
+  cf* d = nullptr;
+  e = hipMalloc(&d, sizeof(cf) * N);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(twiddles)");
+  e = hipMemcpy(d, h.data(), sizeof(cf) * N, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(d);
+    return hip_fail(e, "hipMemcpy(twiddles)");
+  }
+  g_tw[{dev, N}] = d;
+  *out = d;
+  return 0;
+}
+
+// ---------------------------------------------------------------- finalize
+struct FinArgs {
+  const float* ws;
+  void* out0;
+  void* out1;
+  void* out2;
+  int64_t rows;
+  int32_t bpr, N, nbins, nc, kind, fold, apply_factor;
+  double scale;  // scale / nseg
+};
+
+// Sum the block partials of each row in FP64 (fixed order), fold the
+// conjugate mirror of the two-for-one packing, apply scale, 1/nseg and the
+// onesided factor.  Counterpart of _reduce_groups (_stft_metal.py:443-455)
+// plus the deferred factor of csd (spectral.py:482-488).
+__global__ __launch_bounds__(256) void k_finalize(const FinArgs f) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= f.rows * f.nbins) return;
+  const int64_t row = idx / f.nbins;
+  const int k = (int)(idx - row * f.nbins);
+  const int N = f.N;
+  const int mk = (N - k) & (N - 1);
+  const float* base = f.ws + (size_t)row * f.bpr * f.nc * N;
+  double s[4] = {0, 0, 0, 0}, m[4] = {0, 0, 0, 0};
+  for (int b = 0; b < f.bpr; ++b) {
+    const float* pb = base + (size_t)b * f.nc * N;
+    for (int c = 0; c < f.nc; ++c) {
+      s[c] += (double)pb[c * N + k];
+      if (f.fold) m[c] += (double)pb[c * N + mk];
+    }
+  }
+  double p[4];
+  if (f.fold) {
+    // real parts / powers fold symmetric, imaginary parts antisymmetric
+    if (f.nc == 1) {
+      p[0] = 0.5 * (s[0] + m[0]);
+    } else if (f.nc == 2) {
+      p[0] = 0.5 * (s[0] + m[0]);
+      p[1] = 0.5 * (s[1] - m[1]);
+    } else {
+      p[0] = 0.5 * (s[0] + m[0]);
+      p[1] = 0.5 * (s[1] + m[1]);
+      p[2] = 0.5 * (s[2] + m[2]);
+      p[3] = 0.5 * (s[3] - m[3]);
+    }
+  } else {
+    for (int c = 0; c < 4; ++c) p[c] = s[c];
+  }
+  double sc = f.scale;
+  if (f.apply_factor && !(k == 0 || 2 * k == N)) sc *= 2.0;
+  if (f.kind == MSIG_PSD_MEAN) {
+    static_cast<float*>(f.out0)[idx] = (float)(p[0] * sc);
+  } else if (f.kind == MSIG_CSD_MEAN) {
+    static_cast<cf*>(f.out0)[idx] = cf{(float)(p[0] * sc), (float)(p[1] * sc)};
+  } else if (f.kind == MSIG_COH_TRIO_MEAN) {
+    static_cast<float*>(f.out0)[idx] = (float)(p[0] * sc);
+    static_cast<float*>(f.out1)[idx] = (float)(p[1] * sc);
+    static_cast<cf*>(f.out2)[idx] = cf{(float)(p[2] * sc), (float)(p[3] * sc)};
+  } else {  // MSIG_COHERENCE: scales cancel in the ratio
+    const double num = p[2] * p[2] + p[3] * p[3];
+    static_cast<float*>(f.out0)[idx] = (float)(num / p[0] / p[1]);
+  }
+}
+
+// ---------------------------------------------------------------- dispatch
+static bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+static hipError_t dispatch_mean(int N, int kk, bool cplx, const KArgs& a, int64_t rows,
+                                hipStream_t s) {
+  switch (N) {
+#define C(NV) \
+  case NV:    \
+    return launch_mean_##NV(kk, cplx, a, rows, s);
+    C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+  }
+  return hipErrorInvalidValue;
+}
+
+static hipError_t dispatch_frames(int N, int ok, bool cplx, const KArgs& a, int64_t rows,
+                                  hipStream_t s) {
+  switch (N) {
+#define C(NV) \
+  case NV:    \
+    return launch_frames_##NV(ok, cplx, a, rows, s);
+    C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+  }
+  return hipErrorInvalidValue;
+}
+
+static int ncomp(int kind) {
+  switch (kind) {
+    case MSIG_PSD_MEAN: return 1;
+    case MSIG_CSD_MEAN: return 2;
+    default: return 4;
+  }
+}
+
+static bool is_mean(int kind) {
+  return kind == MSIG_PSD_MEAN || kind == MSIG_CSD_MEAN || kind == MSIG_COH_TRIO_MEAN ||
+         kind == MSIG_COHERENCE;
+}
+
+static bool is_pair(int kind) {
+  return kind == MSIG_CSD_MEAN || kind == MSIG_COH_TRIO_MEAN || kind == MSIG_COHERENCE ||
+         kind == MSIG_CSD_FRAMES || kind == MSIG_COH_FRAMES;
+}
+
+static int check_job(const msig_job* j) {
+  if (!j) return fail(MSIG_EINVAL, "job is NULL");
+  if (j->kind < MSIG_PSD_MEAN || j->kind > MSIG_COH_FRAMES)
+    return fail(MSIG_EINVAL, "unknown kind %d", j->kind);
+  if (j->rows < 1) return fail(MSIG_EINVAL, "rows must be >= 1 (got %lld)", (long long)j->rows);
+  if (j->nperseg < 1) return fail(MSIG_EINVAL, "nperseg must be >= 1 (got %d)", j->nperseg);
+  if (j->nstep < 1) return fail(MSIG_EINVAL, "nstep must be >= 1 (got %d)", j->nstep);
+  if (j->nfft < j->nperseg)
+    return fail(MSIG_EINVAL, "nfft (%d) must be >= nperseg (%d)", j->nfft, j->nperseg);
+  if (j->n < j->nperseg)
+    return fail(MSIG_EINVAL, "n (%lld) must be >= nperseg (%d)", (long long)j->n, j->nperseg);
+  if (j->ld_x < j->n) return fail(MSIG_EINVAL, "ld_x < n");
+  if (is_pair(j->kind) && j->ld_y < j->n) return fail(MSIG_EINVAL, "ld_y < n");
+  const int64_t nseg = (j->n - j->nperseg) / j->nstep + 1;
+  if (nseg != j->nseg)
+    return fail(MSIG_EINVAL, "nseg %d != (n - nperseg) / nstep + 1 = %lld", j->nseg,
+                (long long)nseg);
+  if (nseg > (int64_t)1 << 30) return fail(MSIG_EINVAL, "nseg too large");
+  if (!pow2(j->nfft) || j->nfft < MSIG_MIN_NFFT || j->nfft > MSIG_MAX_NFFT)
+    return fail(MSIG_ENOTSUP, "nfft %d outside the fused power-of-two range [%d, %d]", j->nfft,
+                MSIG_MIN_NFFT, MSIG_MAX_NFFT);
+  return 0;
+}
+
+static int nbins_of(const msig_job* j) {
+  const bool cplx = j->flags & MSIG_F_COMPLEX_INPUT;
+  return (!cplx && (j->flags & MSIG_F_ONESIDED)) ? j->nfft / 2 + 1 : j->nfft;
+}
+
+static size_t mean_ws_bytes(const msig_job* j) {
+  const bool cplx = j->flags & MSIG_F_COMPLEX_INPUT;
+  const Geo g = geometry(j->rows, j->nseg, j->nfft, cplx, MEAN_MAX_RUN);
+  return (size_t)j->rows * g.bpr * ncomp(j->kind) * j->nfft * sizeof(float);
+}
+
+static KArgs base_args(const msig_job* j, const void* x, const void* y, const float* win,
+                       const cf* tw, const Geo& g) {
+  KArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.x = x;
+  a.y = y;
+  a.win = win;
+  a.twt = tw;
+  a.ld_x = j->ld_x;
+  a.ld_y = j->ld_y;
+  a.nseg = j->nseg;
+  a.nperseg = j->nperseg;
+  a.hop = j->nstep;
+  a.units = g.units;
+  a.run = g.run;
+  a.bpr = g.bpr;
+  a.detrend = (j->flags & MSIG_F_DETREND_CONST) ? 1 : 0;
+  return a;
+}
+
+}  // namespace msig
+
+using namespace msig;
+
+extern "C" {
+
+int msig_abi_version(void) { return MSIG_ABI_VERSION; }
+
+const char* msig_last_error(void) { return g_err.c_str(); }
+
+int msig_device_count(void) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e == hipErrorNoDevice) return 0;
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+  return n;
+}
+
+int msig_init(int device) {
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  for (int N = MSIG_MIN_NFFT; N <= MSIG_MAX_NFFT; N *= 2) {
+    const cf* t = nullptr;
+    int rc = twiddles(N, &t);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int msig_eligible(int32_t nperseg, int32_t nfft, int32_t onesided, int32_t detrend_const,
+                  int32_t complex_input) {
+  (void)onesided;
+  (void)detrend_const;
+  (void)complex_input;
+  if (nperseg < 1 || nfft < nperseg) return 0;
+  if (!pow2(nfft) || nfft < MSIG_MIN_NFFT || nfft > MSIG_MAX_NFFT) return 0;
+  return 1;
+}
+
+int msig_workspace_bytes(const msig_job* job, size_t* bytes) {
+  int rc = check_job(job);
+  if (rc) return rc;
+  if (!bytes) return fail(MSIG_EINVAL, "bytes is NULL");
+  *bytes = is_mean(job->kind) ? mean_ws_bytes(job) : 0;
+  return 0;
+}
+
+int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const float* win,
+                       void* out0, void* out1, void* out2, void* workspace, size_t ws_bytes,
+                       void* stream) {
+  int rc = check_job(job);
+  if (rc) return rc;
+  if (!is_mean(job->kind)) return fail(MSIG_EINVAL, "kind %d is not a mean kind", job->kind);
+  const bool pair = is_pair(job->kind);
+  if (!x || !win || !out0 || (pair && !y)) return fail(MSIG_EINVAL, "NULL input/output pointer");
+  if (job->kind == MSIG_COH_TRIO_MEAN && (!out1 || !out2))
+    return fail(MSIG_EINVAL, "coherence trio needs three outputs");
+  const size_t need = mean_ws_bytes(job);
+  if (!workspace || ws_bytes < need)
+    return fail(MSIG_EINVAL, "workspace of %zu bytes needed (got %zu)", need, ws_bytes);
+  const bool cplx = job->flags & MSIG_F_COMPLEX_INPUT;
+  const int N = job->nfft;
+  const Geo g = geometry(job->rows, job->nseg, N, cplx, MEAN_MAX_RUN);
+  const cf* tw = nullptr;
+  rc = twiddles(N, &tw);
+  if (rc) return rc;
+  KArgs a = base_args(job, x, y, win, tw, g);
+  a.ws = static_cast<float*>(workspace);
+  const int kk = job->kind == MSIG_PSD_MEAN ? K_PSD : (job->kind == MSIG_CSD_MEAN ? K_CSD : K_COH);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = dispatch_mean(N, kk, cplx, a, job->rows, s);
+  if (e != hipSuccess) return hip_fail(e, "fused mean kernel launch");
+  FinArgs f;
+  f.ws = a.ws;
+  f.out0 = out0;
+  f.out1 = out1;
+  f.out2 = out2;
+  f.rows = job->rows;
+  f.bpr = g.bpr;
+  f.N = N;
+  f.nbins = nbins_of(job);
+  f.nc = ncomp(job->kind);
+  f.kind = job->kind;
+  f.fold = cplx ? 0 : 1;
+  f.apply_factor = (!cplx && (job->flags & MSIG_F_APPLY_FACTOR) && (job->flags & MSIG_F_ONESIDED)) ? 1 : 0;
+  f.scale = job->scale / (double)job->nseg;
+  const int64_t total = job->rows * f.nbins;
+  const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+  hipLaunchKernelGGL(k_finalize, grid, block, 0, s, f);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "finalize kernel launch");
+  return 0;
+}
+
+int msig_spectral_frames(const msig_job* job, const void* x, const void* y, const float* win,
+                         void* out0, void* out1, void* out2, void* stream) {
+  int rc = check_job(job);
+  if (rc) return rc;
+  if (is_mean(job->kind)) return fail(MSIG_EINVAL, "kind %d is not a per-frame kind", job->kind);
+  const bool pair = is_pair(job->kind);
+  if (!x || !win || !out0 || (pair && !y)) return fail(MSIG_EINVAL, "NULL input/output pointer");
+  if (job->kind == MSIG_COH_FRAMES && (!out1 || !out2))
+    return fail(MSIG_EINVAL, "coherence frames need three outputs");
+  const bool cplx = job->flags & MSIG_F_COMPLEX_INPUT;
+  const int N = job->nfft;
+  const Geo g = geometry(job->rows, job->nseg, N, cplx, FRAMES_MAX_RUN);
+  const cf* tw = nullptr;
+  rc = twiddles(N, &tw);
+  if (rc) return rc;
+  KArgs a = base_args(job, x, y, win, tw, g);
+  a.out0 = out0;
+  a.out1 = out1;
+  a.out2 = out2;
+  a.nbins = nbins_of(job);
+  a.apply_factor =
+      (!cplx && (job->flags & MSIG_F_APPLY_FACTOR) && (job->flags & MSIG_F_ONESIDED)) ? 1 : 0;
+  int ok = F_COMPLEX;
+  switch (job->kind) {
+    case MSIG_FRAMES_COMPLEX: ok = F_COMPLEX; break;
+    case MSIG_FRAMES_POWER: ok = F_POWER; break;
+    case MSIG_CSD_FRAMES: ok = F_CSD; break;
+    default: ok = F_COH; break;
+  }
+  a.oscale = ok == F_COMPLEX ? (float)std::sqrt(job->scale) : (float)job->scale;
+  hipError_t e = dispatch_frames(N, ok, cplx, a, job->rows, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "fused frames kernel launch");
+  return 0;
+}
+
+// ---- operator-level entry points (reference _stft_metal.py functions) ----
+
+static msig_job op_job(int kind, int64_t rows, int64_t n, int32_t nperseg, int32_t hop,
+                       int32_t detrend) {
+  msig_job j;
+  std::memset(&j, 0, sizeof j);
+  j.kind = kind;
+  j.flags = MSIG_F_ONESIDED | (detrend ? MSIG_F_DETREND_CONST : 0);
+  j.rows = rows;
+  j.n = n;
+  j.ld_x = n;
+  j.ld_y = n;
+  j.nperseg = nperseg;
+  j.nstep = hop;
+  j.nfft = nperseg;
+  j.nseg = (hop >= 1 && n >= nperseg) ? (int32_t)((n - nperseg) / hop + 1) : 0;
+  j.scale = 1.0;
+  return j;
+}
+
+int msig_rfft_frames(const float* x, int64_t rows, int64_t n, const float* win, int32_t nperseg,
+                     int32_t hop, int32_t detrend, int32_t power, void* out, void* stream) {
+  msig_job j = op_job(power ? MSIG_FRAMES_POWER : MSIG_FRAMES_COMPLEX, rows, n, nperseg, hop, detrend);
+  return msig_spectral_frames(&j, x, nullptr, win, out, nullptr, nullptr, stream);
+}
+
+int msig_rfft_power_mean(const float* x, int64_t rows, int64_t n, const float* win,
+                         int32_t nperseg, int32_t hop, int32_t detrend, float* out,
+                         void* workspace, size_t ws_bytes, void* stream) {
+  msig_job j = op_job(MSIG_PSD_MEAN, rows, n, nperseg, hop, detrend);
+  return msig_spectral_mean(&j, x, nullptr, win, out, nullptr, nullptr, workspace, ws_bytes, stream);
+}
+
+int msig_csd_frames(const float* x, const float* y, int64_t rows, int64_t n, const float* win,
+                    int32_t nperseg, int32_t hop, int32_t detrend, void* out, void* stream) {
+  msig_job j = op_job(MSIG_CSD_FRAMES, rows, n, nperseg, hop, detrend);
+  return msig_spectral_frames(&j, x, y, win, out, nullptr, nullptr, stream);
+}
+
+int msig_coherence_frames(const float* x, const float* y, int64_t rows, int64_t n,
+                          const float* win, int32_t nperseg, int32_t hop, int32_t detrend,
+                          float* pxx, float* pyy, void* pxy, void* stream) {
+  msig_job j = op_job(MSIG_COH_FRAMES, rows, n, nperseg, hop, detrend);
+  return msig_spectral_frames(&j, x, y, win, pxx, pyy, pxy, stream);
+}
+
+int msig_csd_mean(const float* x, const float* y, int64_t rows, int64_t n, const float* win,
+                  int32_t nperseg, int32_t hop, int32_t detrend, void* out, void* workspace,
+                  size_t ws_bytes, void* stream) {
+  msig_job j = op_job(MSIG_CSD_MEAN, rows, n, nperseg, hop, detrend);
+  return msig_spectral_mean(&j, x, y, win, out, nullptr, nullptr, workspace, ws_bytes, stream);
+}
+
+int msig_coherence_mean(const float* x, const float* y, int64_t rows, int64_t n,
+                        const float* win, int32_t nperseg, int32_t hop, int32_t detrend,
+                        float* pxx, float* pyy, void* pxy, void* workspace, size_t ws_bytes,
+                        void* stream) {
+  msig_job j = op_job(MSIG_COH_TRIO_MEAN, rows, n, nperseg, hop, detrend);
+  return msig_spectral_mean(&j, x, y, win, pxx, pyy, pxy, workspace, ws_bytes, stream);
+}
+
+int msig_mean_workspace_bytes(int32_t kind, int64_t rows, int64_t n, int32_t nperseg, int32_t hop,
+                              size_t* bytes) {
+  if (kind != MSIG_PSD_MEAN && kind != MSIG_CSD_MEAN && kind != MSIG_COH_TRIO_MEAN)
+    return fail(MSIG_EINVAL, "kind %d has no mean workspace", kind);
+  msig_job j = op_job(kind, rows, n, nperseg, hop, 0);
+  return msig_workspace_bytes(&j, bytes);
+}
+
+}  // extern "C"

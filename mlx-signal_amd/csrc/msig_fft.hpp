@@ -1,0 +1,256 @@
+// Register-resident Stockham FFT building blocks for the fused spectral kernels.
+//
+// One FFT of size N is carried by a "group" of TPF = N / E threads, each
+// holding E complex points in registers.  Pass p of the radix schedule reads
+// R_p points at stride N / R_p, applies the Stockham twiddles, runs an
+// in-register DFT_R and writes its outputs back through LDS at the
+// self-sorting positions, so the final pass leaves thread t holding bins
+// t + TPF * e (natural order, no bit reversal).  The same templates compile
+// for the host, which lets the index math be exercised lane by lane on a CPU
+// (msig_fft_emu.cpp) before any GPU run.
+//
+// The real-input forward transform replaces the reference's per-segment
+// even/odd packing + radix-2 Stockham in threadgroup memory
+// (reference: src/mlx_signal_processing/_stft_metal.py:80-147); here two real
+// segments ride in one complex FFT (z = a + i b) and are separated later with
+// the conjugate-mirror identity, which needs no untangle twiddles at all.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MSIG_HD __host__ __device__ __forceinline__
+#else
+#define MSIG_HD inline
+#endif
+
+namespace msig {
+
+struct cf {
+  float re, im;
+};
+
+MSIG_HD cf cadd(cf a, cf b) { return cf{a.re + b.re, a.im + b.im}; }
+MSIG_HD cf csub(cf a, cf b) { return cf{a.re - b.re, a.im - b.im}; }
+// a * b
+MSIG_HD cf cmul(cf a, cf b) {
+  return cf{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// conj(a) * b
+MSIG_HD cf cmulc(cf a, cf b) {
+  return cf{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+
+// cos(2*pi*m/64), float32-rounded from FP64, m = 0..63.
+MSIG_HD float cos64(int m) {
+  constexpr float t[64] = {
+      1.0f,          0.99518472f,   0.980785251f,  0.956940353f,  0.923879504f,
+      0.881921291f,  0.831469595f,  0.773010433f,  0.707106769f,  0.634393275f,
+      0.555570245f,  0.471396744f,  0.382683426f,  0.290284663f,  0.195090324f,
+      0.0980171412f, 0.0f,          -0.0980171412f, -0.195090324f, -0.290284663f,
+      -0.382683426f, -0.471396744f, -0.555570245f, -0.634393275f, -0.707106769f,
+      -0.773010433f, -0.831469595f, -0.881921291f, -0.923879504f, -0.956940353f,
+      -0.980785251f, -0.99518472f,  -1.0f,         -0.99518472f,  -0.980785251f,
+      -0.956940353f, -0.923879504f, -0.881921291f, -0.831469595f, -0.773010433f,
+      -0.707106769f, -0.634393275f, -0.555570245f, -0.471396744f, -0.382683426f,
+      -0.290284663f, -0.195090324f, -0.0980171412f, 0.0f,          0.0980171412f,
+      0.195090324f,  0.290284663f,  0.382683426f,  0.471396744f,  0.555570245f,
+      0.634393275f,  0.707106769f,  0.773010433f,  0.831469595f,  0.881921291f,
+      0.923879504f,  0.956940353f,  0.980785251f,  0.99518472f};
+  return t[m & 63];
+}
+
+// v * exp(-2*pi*i*m/R) for a compile-time-foldable (m, R), R <= 64.
+// Multiples of a quarter turn become swaps/negations; the rest cost one
+// complex multiply against constants.
+MSIG_HD cf mul_wconst(cf v, int m, int R) {
+  m = ((m % R) + R) % R;
+  if (m == 0) return v;
+  if (4 * m == R) return cf{v.im, -v.re};       // * (-i)
+  if (2 * m == R) return cf{-v.re, -v.im};      // * (-1)
+  if (4 * m == 3 * R) return cf{-v.im, v.re};   // * (+i)
+  const int q = m * (64 / R);
+  const float c = cos64(q);
+  const float s = cos64(q - 16);                // sin(2*pi*q/64)
+  // (re + i im)(c - i s)
+  return cf{v.re * c + v.im * s, v.im * c - v.re * s};
+}
+
+// In-register forward DFT of size R (natural order in and out):
+// V[k] = sum_j v[j] exp(-2*pi*i*j*k/R).
+template <int R>
+struct Dft;
+
+template <>
+struct Dft<1> {
+  static MSIG_HD void run(cf*) {}
+};
+
+template <>
+struct Dft<2> {
+  static MSIG_HD void run(cf* v) {
+    cf a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <>
+struct Dft<4> {
+  static MSIG_HD void run(cf* v) {
+    cf a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    cf c = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cf{b.re + d.im, b.im - d.re};  // b - i d
+    v[3] = cf{b.re - d.im, b.im + d.re};  // b + i d
+  }
+};
+
+// Composite sizes: Cooley-Tukey R = R1 * R2 entirely in registers,
+// n = R2*n1 + n2, k = k1 + R1*k2.
+template <int R>
+struct Dft {
+  static constexpr int R1 = (R >= 16) ? 4 : 2;
+  static constexpr int R2 = R / R1;
+  static MSIG_HD void run(cf* v) {
+    cf a[R2][R1];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) a[n2][n1] = v[R2 * n1 + n2];
+      Dft<R1>::run(a[n2]);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) a[n2][k1] = mul_wconst(a[n2][k1], n2 * k1, R);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      cf b[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) b[n2] = a[n2][k1];
+      Dft<R2>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+    }
+  }
+};
+
+// Radix schedule for an N-point FFT carried E points per thread:
+// E, E, ..., remainder.  N and E are powers of two, E <= N.
+template <int N, int E>
+struct Plan {
+  static_assert((N & (N - 1)) == 0 && (E & (E - 1)) == 0 && E <= N, "pow2 sizes");
+  static constexpr int TPF = N / E;  // threads per FFT
+  static constexpr int radix(int p) {
+    int rem = N;
+    for (int i = 0; i < p; ++i) rem /= (rem < E ? rem : E);
+    return rem < E ? rem : E;
+  }
+  static constexpr int npasses() {
+    int rem = N, c = 0;
+    while (rem > 1) {
+      rem /= (rem < E ? rem : E);
+      ++c;
+    }
+    return c;
+  }
+  static constexpr int ns(int p) {
+    int s = 1;
+    for (int i = 0; i < p; ++i) s *= radix(i);
+    return s;
+  }
+  static constexpr int NP = npasses();
+  // twiddle registers held per thread: passes p >= 1, (E/R)*(R-1) each
+  static constexpr int tw_off(int p) {
+    int o = 0;
+    for (int i = 1; i < p; ++i) o += (E / radix(i)) * (radix(i) - 1);
+    return o;
+  }
+  static constexpr int TWN = tw_off(NP) > 0 ? tw_off(NP) : 1;
+  // LDS slot of logical position pos: one pad slot every 16 keeps the
+  // stride-16 pass-0 stores and the stride-1 reads bank-conflict free.
+  static MSIG_HD int pad(int pos) { return pos + (pos >> 4); }
+  static constexpr int LDS = N + N / 16;
+  // final bin held by thread t in register e
+  static MSIG_HD int bin(int t, int e) {
+    constexpr int R = radix(NP - 1);
+    constexpr int NS = ns(NP - 1);
+    const int u = e / R, j = e % R;
+    return t + TPF * u + NS * j;
+  }
+  // logical position written by thread t, register e, after pass p
+  template <int P>
+  static MSIG_HD int out_pos(int t, int e) {
+    constexpr int R = radix(P);
+    constexpr int NS = ns(P);
+    const int u = e / R, j = e % R;
+    const int vt = t + TPF * u;
+    return (vt / NS) * NS * R + (vt % NS) + j * NS;
+  }
+  // logical position read by thread t into register e for pass P
+  template <int P>
+  static MSIG_HD int in_pos(int t, int e) {
+    constexpr int R = radix(P);
+    const int u = e / R, j = e % R;
+    return t + TPF * u + (N / R) * j;
+  }
+  // twiddle exponent (units of exp(-2*pi*i/N)) for pass P, register e
+  template <int P>
+  static MSIG_HD int tw_exp(int t, int e) {
+    constexpr int R = radix(P);
+    constexpr int NS = ns(P);
+    const int u = e / R, j = e % R;
+    const int vt = t + TPF * u;
+    return (j * (vt % NS) * (N / (NS * R))) % N;
+  }
+};
+
+// Load the per-thread twiddle registers from an N-entry table of
+// exp(-2*pi*i*e/N) (computed in FP64 by the host, stored as float pairs).
+template <int N, int E, int P = 1>
+MSIG_HD void load_twiddles(int t, const cf* table, cf* tw) {
+  using PL = Plan<N, E>;
+  if constexpr (P < PL::NP) {
+    constexpr int R = PL::radix(P);
+    constexpr int off = PL::tw_off(P);
+#pragma unroll
+    for (int u = 0; u < E / R; ++u)
+#pragma unroll
+      for (int j = 1; j < R; ++j)
+        tw[off + u * (R - 1) + (j - 1)] = table[PL::template tw_exp<P>(t, u * R + j)];
+    load_twiddles<N, E, P + 1>(t, table, tw);
+  }
+}
+
+// Pass P compute step on registers already read for that pass.
+template <int N, int E, int P>
+MSIG_HD void pass_compute(cf* v, const cf* tw) {
+  using PL = Plan<N, E>;
+  constexpr int R = PL::radix(P);
+#pragma unroll
+  for (int u = 0; u < E / R; ++u) {
+    if constexpr (P > 0) {
+      constexpr int off = PL::tw_off(P);
+#pragma unroll
+      for (int j = 1; j < R; ++j) v[u * R + j] = cmul(v[u * R + j], tw[off + u * (R - 1) + (j - 1)]);
+    }
+    Dft<R>::run(v + u * R);
+  }
+}
+
+template <int N, int E, int P>
+MSIG_HD void pass_store(int t, const cf* v, cf* lds) {
+  using PL = Plan<N, E>;
+#pragma unroll
+  for (int e = 0; e < E; ++e) lds[PL::pad(PL::template out_pos<P>(t, e))] = v[e];
+}
+
+template <int N, int E, int P>
+MSIG_HD void pass_load(int t, cf* v, const cf* lds) {
+  using PL = Plan<N, E>;
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = lds[PL::pad(PL::template in_pos<P>(t, e))];
+}
+
+}  // namespace msig

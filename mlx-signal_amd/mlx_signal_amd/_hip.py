@@ -1,0 +1,126 @@
+"""ctypes binding of libmsig.so, the gfx950 kernels' C ABI (include/msig.h).
+
+The library is built in-tree (``python mlx-signal_amd/build.py``) and loaded
+from this directory; there is no other implementation to fall back to, so a
+missing or stale library raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).with_name("libmsig.so")
+ABI_VERSION = 1
+
+MSIG_EINVAL = -22
+MSIG_ENOTSUP = -95
+MSIG_EIO = -5
+
+MIN_NFFT = 16
+MAX_NFFT = 4096
+
+# kinds
+PSD_MEAN, CSD_MEAN, COH_TRIO_MEAN, COHERENCE = 0, 1, 2, 3
+FRAMES_COMPLEX, FRAMES_POWER, CSD_FRAMES, COH_FRAMES = 4, 5, 6, 7
+
+# flags
+F_COMPLEX_INPUT = 0x1
+F_ONESIDED = 0x2
+F_APPLY_FACTOR = 0x4
+F_DETREND_CONST = 0x8
+
+
+class MsigJob(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("rows", ctypes.c_int64),
+        ("n", ctypes.c_int64),
+        ("ld_x", ctypes.c_int64),
+        ("ld_y", ctypes.c_int64),
+        ("nperseg", ctypes.c_int32),
+        ("nstep", ctypes.c_int32),
+        ("nfft", ctypes.c_int32),
+        ("nseg", ctypes.c_int32),
+        ("scale", ctypes.c_double),
+    ]
+
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_sz = ctypes.c_size_t
+_JOBP = ctypes.POINTER(MsigJob)
+
+# name -> (restype, argtypes), exactly the declarations of include/msig.h
+SIGNATURES = {
+    "msig_abi_version": (ctypes.c_int, []),
+    "msig_last_error": (ctypes.c_char_p, []),
+    "msig_device_count": (ctypes.c_int, []),
+    "msig_init": (ctypes.c_int, [ctypes.c_int]),
+    "msig_eligible": (ctypes.c_int, [_i32, _i32, _i32, _i32, _i32]),
+    "msig_workspace_bytes": (ctypes.c_int, [_JOBP, ctypes.POINTER(_sz)]),
+    "msig_spectral_mean": (ctypes.c_int, [_JOBP, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "msig_spectral_frames": (ctypes.c_int, [_JOBP, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "msig_rfft_frames": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i32, _i32, _i32, _i32, _vp, _vp]),
+    "msig_rfft_power_mean": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i32, _i32, _i32, _vp, _vp, _sz, _vp]),
+    "msig_csd_frames": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _i32, _i32, _i32, _vp, _vp]),
+    "msig_coherence_frames": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
+    "msig_csd_mean": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _i32, _i32, _i32, _vp, _vp, _sz, _vp]),
+    "msig_coherence_mean": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "msig_mean_workspace_bytes": (ctypes.c_int, [_i32, _i64, _i64, _i32, _i32, ctypes.POINTER(_sz)]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load() -> ctypes.CDLL:
+    """Load libmsig.so once and bind every entry point's prototype."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python mlx-signal_amd/build.py` "
+                "(the fused spectral path has no non-HIP implementation)"
+            )
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        v = lib.msig_abi_version()
+        if v != ABI_VERSION:
+            raise RuntimeError(f"libmsig.so ABI {v} != expected {ABI_VERSION}: rebuild it")
+        _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    """Turn a status code into the exception the reference would raise."""
+    if rc == 0:
+        return
+    msg = load().msig_last_error().decode(errors="replace")
+    text = f"{what}: {msg}"
+    if rc == MSIG_EINVAL:
+        raise ValueError(text)
+    if rc == MSIG_ENOTSUP:
+        raise NotImplementedError(text)
+    raise RuntimeError(text)
+
+
+def stream_of(t: torch.Tensor):
+    """The current HIP stream of t's device, as the raw handle the ABI takes."""
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())

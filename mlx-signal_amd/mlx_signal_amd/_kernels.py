@@ -1,0 +1,293 @@
+"""Fused frame+detrend+window+FFT(+power / cross / mean) operators on gfx950.
+
+Counterpart of the reference's fused-kernel module
+(src/mlx_signal_processing/_stft_metal.py): the same operator names and
+argument meanings (``eligible``, ``rfft_frames``, ``rfft_power_mean``,
+``csd_frames``, ``coherence_frames``, ``csd_mean``, ``coherence_mean``), each
+one call into libmsig.so on the tensors' current HIP stream.  The generic
+``spectral_mean`` / ``spectral_frames`` carry the extra options the planner
+uses (raw window + scale applied in FP64, in-kernel onesided factor,
+nfft > nperseg zero padding, two-sided and complex input).
+
+Tensors passed in must live on a ROCm device; outputs are allocated there.
+"""
+
+from __future__ import annotations
+
+import collections
+import ctypes
+
+import torch
+
+from . import _hip
+
+#: launches per kind name, so tests can assert the fused path ran
+launches: collections.Counter = collections.Counter()
+
+_MEAN_KINDS = {
+    "psd": _hip.PSD_MEAN,
+    "csd": _hip.CSD_MEAN,
+    "coh_trio": _hip.COH_TRIO_MEAN,
+    "coherence": _hip.COHERENCE,
+}
+_FRAME_KINDS = {
+    "complex": _hip.FRAMES_COMPLEX,
+    "power": _hip.FRAMES_POWER,
+    "csd": _hip.CSD_FRAMES,
+    "coh": _hip.COH_FRAMES,
+}
+
+
+def _is_const(detrend) -> bool:
+    return isinstance(detrend, str) and detrend in ("constant", "c")
+
+
+def eligible(x, nperseg: int, nfft: int, sides: str, detrend) -> bool:
+    """True if the fused kernels compute this windowed FFT
+    (reference: _stft_metal.py:150-160; widened to nfft >= nperseg powers of
+    two in [16, 4096], two-sided output and complex input)."""
+    if not isinstance(x, torch.Tensor) or x.device.type != "cuda":
+        return False
+    if x.dtype not in (torch.float32, torch.complex64):
+        return False
+    if not (detrend is False or detrend is None or _is_const(detrend)):
+        return False
+    cplx = x.dtype == torch.complex64
+    if cplx and sides == "onesided":
+        return False
+    lib = _hip.load()
+    return bool(lib.msig_eligible(int(nperseg), int(nfft), int(sides == "onesided"),
+                                  int(_is_const(detrend)), int(cplx)))
+
+
+def _rows(t: torch.Tensor) -> torch.Tensor:
+    if t.stride(-1) != 1 or (t.dim() == 2 and t.stride(0) < t.shape[1]):
+        t = t.contiguous()
+    return t
+
+
+def _job(kind, x2, y2, nperseg, nstep, nfft, detrend, onesided, apply_factor, scale):
+    rows, n = x2.shape
+    cplx = x2.dtype == torch.complex64
+    flags = (_hip.F_COMPLEX_INPUT if cplx else 0) | (_hip.F_ONESIDED if onesided else 0)
+    flags |= _hip.F_APPLY_FACTOR if apply_factor else 0
+    flags |= _hip.F_DETREND_CONST if _is_const(detrend) else 0
+    nseg = (n - nperseg) // nstep + 1
+    ld_y = y2.stride(0) if y2 is not None else n
+    return _hip.MsigJob(kind, flags, rows, n, x2.stride(0), ld_y, nperseg, nstep, nfft, nseg,
+                        float(scale))
+
+
+def _check_inputs(x2, y2, win, nperseg):
+    if x2.dim() != 2 or x2.device.type != "cuda":
+        raise ValueError("fused kernels take (rows, n) tensors on a ROCm device")
+    if y2 is not None and (y2.shape != x2.shape or y2.dtype != x2.dtype or y2.device != x2.device):
+        raise ValueError("x and y must have the same shape, dtype and device")
+    if win.dtype != torch.float32 or win.numel() != nperseg or not win.is_contiguous():
+        raise ValueError("window must be a contiguous float32 tensor of length nperseg")
+    if win.device != x2.device:
+        raise ValueError("window must live on the input's device")
+
+
+def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, onesided=True,
+                  apply_factor=False, scale=1.0):
+    """Mean over segments of |X|^2 ("psd"), conj(X) Y ("csd"), the
+    (Pxx, Pyy, Pxy) trio ("coh_trio") or |Pxy|^2/(Pxx Pyy) ("coherence"),
+    times ``scale`` (and the onesided factor when ``apply_factor``).
+    x2 / y2: (rows, n).  Returns a tuple of (rows, bins) tensors."""
+    x2 = _rows(x2)
+    y2 = None if y2 is None else _rows(y2)
+    _check_inputs(x2, y2, win, nperseg)
+    code = _MEAN_KINDS[kind]
+    if code != _hip.PSD_MEAN and y2 is None:
+        raise ValueError(f"{kind} needs two signals")
+    lib = _hip.load()
+    job = _job(code, x2, y2, nperseg, nstep, nfft, detrend, onesided, apply_factor, scale)
+    cplx = x2.dtype == torch.complex64
+    bins = nfft // 2 + 1 if (onesided and not cplx) else nfft
+    rows = x2.shape[0]
+    dev = x2.device
+    ws_bytes = ctypes.c_size_t(0)
+    _hip.check(lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(ws_bytes)), kind)
+    ws = torch.empty(max(1, ws_bytes.value), dtype=torch.uint8, device=dev)
+    f32 = dict(dtype=torch.float32, device=dev)
+    c64 = dict(dtype=torch.complex64, device=dev)
+    if code == _hip.PSD_MEAN or code == _hip.COHERENCE:
+        outs = (torch.empty(rows, bins, **f32),)
+    elif code == _hip.CSD_MEAN:
+        outs = (torch.empty(rows, bins, **c64),)
+    else:
+        outs = (torch.empty(rows, bins, **f32), torch.empty(rows, bins, **f32),
+                torch.empty(rows, bins, **c64))
+    o = list(outs) + [None] * (3 - len(outs))
+    rc = lib.msig_spectral_mean(ctypes.byref(job), _hip.ptr(x2), _hip.ptr(y2), _hip.ptr(win),
+                                _hip.ptr(o[0]), _hip.ptr(o[1]), _hip.ptr(o[2]), _hip.ptr(ws),
+                                ws_bytes.value, _hip.stream_of(x2))
+    _hip.check(rc, f"spectral_mean({kind})")
+    launches["mean_" + kind] += 1
+    return outs
+
+
+def spectral_frames(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, onesided=True,
+                    apply_factor=False, scale=1.0):
+    """Per-segment X ("complex", times sqrt(scale)), |X|^2 ("power"),
+    conj(X) Y ("csd") or the trio ("coh"), times ``scale``.  Returns
+    (rows, nseg, bins) tensors."""
+    x2 = _rows(x2)
+    y2 = None if y2 is None else _rows(y2)
+    _check_inputs(x2, y2, win, nperseg)
+    code = _FRAME_KINDS[kind]
+    if code in (_hip.CSD_FRAMES, _hip.COH_FRAMES) and y2 is None:
+        raise ValueError(f"{kind} frames need two signals")
+    lib = _hip.load()
+    job = _job(code, x2, y2, nperseg, nstep, nfft, detrend, onesided, apply_factor, scale)
+    cplx = x2.dtype == torch.complex64
+    bins = nfft // 2 + 1 if (onesided and not cplx) else nfft
+    rows = x2.shape[0]
+    nseg = job.nseg
+    dev = x2.device
+    f32 = dict(dtype=torch.float32, device=dev)
+    c64 = dict(dtype=torch.complex64, device=dev)
+    if code == _hip.FRAMES_POWER:
+        outs = (torch.empty(rows, nseg, bins, **f32),)
+    elif code in (_hip.FRAMES_COMPLEX, _hip.CSD_FRAMES):
+        outs = (torch.empty(rows, nseg, bins, **c64),)
+    else:
+        outs = (torch.empty(rows, nseg, bins, **f32), torch.empty(rows, nseg, bins, **f32),
+                torch.empty(rows, nseg, bins, **c64))
+    o = list(outs) + [None] * (3 - len(outs))
+    rc = lib.msig_spectral_frames(ctypes.byref(job), _hip.ptr(x2), _hip.ptr(y2), _hip.ptr(win),
+                                  _hip.ptr(o[0]), _hip.ptr(o[1]), _hip.ptr(o[2]),
+                                  _hip.stream_of(x2))
+    _hip.check(rc, f"spectral_frames({kind})")
+    launches["frames_" + kind] += 1
+    return outs
+
+
+# ---------------------------------------------------------------------------
+# operator-level functions with the reference's signatures; each is one call
+# of the matching named C entry point
+# ---------------------------------------------------------------------------
+
+
+def _flat(x):
+    n = x.shape[-1]
+    return x.reshape(-1, n).contiguous(), x.shape[:-1], n
+
+
+def _ws(kind, rows, n, nperseg, hop, dev):
+    sz = ctypes.c_size_t(0)
+    lib = _hip.load()
+    _hip.check(lib.msig_mean_workspace_bytes(kind, rows, n, nperseg, hop, ctypes.byref(sz)),
+               "workspace")
+    return torch.empty(max(1, sz.value), dtype=torch.uint8, device=dev), sz.value
+
+
+def rfft_frames(x, win, nperseg: int, hop: int, detrend, power: bool):
+    """(..., nseg, nperseg//2+1) windowed (optionally mean-detrended) rfft of
+    every segment, complex64, or float32 |X|^2 when ``power``
+    (reference: _stft_metal.py:192-220).  ``win`` carries any scale."""
+    x2, batch, n = _flat(x)
+    _check_inputs(x2, None, win, nperseg)
+    nseg = (n - nperseg) // hop + 1
+    m1 = nperseg // 2 + 1
+    out = torch.empty(x2.shape[0], nseg, m1, device=x2.device,
+                      dtype=torch.float32 if power else torch.complex64)
+    lib = _hip.load()
+    rc = lib.msig_rfft_frames(_hip.ptr(x2), x2.shape[0], n, _hip.ptr(win), nperseg, hop,
+                              int(_is_const(detrend)), int(bool(power)), _hip.ptr(out),
+                              _hip.stream_of(x2))
+    _hip.check(rc, "rfft_frames")
+    launches["op_rfft_frames"] += 1
+    return out.reshape(*batch, nseg, m1)
+
+
+def rfft_power_mean(x, win, nperseg: int, hop: int, detrend):
+    """(..., nperseg//2+1) mean windowed |rfft|^2 over segments, without the
+    onesided factor (reference: _stft_metal.py:530-557)."""
+    x2, batch, n = _flat(x)
+    _check_inputs(x2, None, win, nperseg)
+    m1 = nperseg // 2 + 1
+    out = torch.empty(x2.shape[0], m1, device=x2.device, dtype=torch.float32)
+    ws, wsb = _ws(_hip.PSD_MEAN, x2.shape[0], n, nperseg, hop, x2.device)
+    lib = _hip.load()
+    rc = lib.msig_rfft_power_mean(_hip.ptr(x2), x2.shape[0], n, _hip.ptr(win), nperseg, hop,
+                                  int(_is_const(detrend)), _hip.ptr(out), _hip.ptr(ws), wsb,
+                                  _hip.stream_of(x2))
+    _hip.check(rc, "rfft_power_mean")
+    launches["op_rfft_power_mean"] += 1
+    return out.reshape(*batch, m1)
+
+
+def csd_frames(x, y, win, nperseg: int, hop: int, detrend):
+    """conj(F(x_seg)) * F(y_seg) for all segments (reference: _stft_metal.py:411-413)."""
+    x2, batch, n = _flat(x)
+    y2, _, _ = _flat(y)
+    _check_inputs(x2, y2, win, nperseg)
+    nseg = (n - nperseg) // hop + 1
+    m1 = nperseg // 2 + 1
+    out = torch.empty(x2.shape[0], nseg, m1, device=x2.device, dtype=torch.complex64)
+    lib = _hip.load()
+    rc = lib.msig_csd_frames(_hip.ptr(x2), _hip.ptr(y2), x2.shape[0], n, _hip.ptr(win), nperseg,
+                             hop, int(_is_const(detrend)), _hip.ptr(out), _hip.stream_of(x2))
+    _hip.check(rc, "csd_frames")
+    launches["op_csd_frames"] += 1
+    return out.reshape(*batch, nseg, m1)
+
+
+def coherence_frames(x, y, win, nperseg: int, hop: int, detrend):
+    """(|Fx|^2, |Fy|^2, conj(Fx) Fy) per segment (reference: _stft_metal.py:416-418)."""
+    x2, batch, n = _flat(x)
+    y2, _, _ = _flat(y)
+    _check_inputs(x2, y2, win, nperseg)
+    nseg = (n - nperseg) // hop + 1
+    m1 = nperseg // 2 + 1
+    rows = x2.shape[0]
+    pxx = torch.empty(rows, nseg, m1, device=x2.device, dtype=torch.float32)
+    pyy = torch.empty_like(pxx)
+    pxy = torch.empty(rows, nseg, m1, device=x2.device, dtype=torch.complex64)
+    lib = _hip.load()
+    rc = lib.msig_coherence_frames(_hip.ptr(x2), _hip.ptr(y2), rows, n, _hip.ptr(win), nperseg,
+                                   hop, int(_is_const(detrend)), _hip.ptr(pxx), _hip.ptr(pyy),
+                                   _hip.ptr(pxy), _hip.stream_of(x2))
+    _hip.check(rc, "coherence_frames")
+    launches["op_coherence_frames"] += 1
+    shp = (*batch, nseg, m1)
+    return pxx.reshape(shp), pyy.reshape(shp), pxy.reshape(shp)
+
+
+def csd_mean(x, y, win, nperseg: int, hop: int, detrend):
+    """Mean conj(F(x_seg)) * F(y_seg) over segments (reference: _stft_metal.py:739-741)."""
+    x2, batch, n = _flat(x)
+    y2, _, _ = _flat(y)
+    _check_inputs(x2, y2, win, nperseg)
+    m1 = nperseg // 2 + 1
+    out = torch.empty(x2.shape[0], m1, device=x2.device, dtype=torch.complex64)
+    ws, wsb = _ws(_hip.CSD_MEAN, x2.shape[0], n, nperseg, hop, x2.device)
+    lib = _hip.load()
+    rc = lib.msig_csd_mean(_hip.ptr(x2), _hip.ptr(y2), x2.shape[0], n, _hip.ptr(win), nperseg,
+                           hop, int(_is_const(detrend)), _hip.ptr(out), _hip.ptr(ws), wsb,
+                           _hip.stream_of(x2))
+    _hip.check(rc, "csd_mean")
+    launches["op_csd_mean"] += 1
+    return out.reshape(*batch, m1)
+
+
+def coherence_mean(x, y, win, nperseg: int, hop: int, detrend):
+    """Mean (|Fx|^2, |Fy|^2, conj(Fx) Fy) over segments (reference: _stft_metal.py:744-746)."""
+    x2, batch, n = _flat(x)
+    y2, _, _ = _flat(y)
+    _check_inputs(x2, y2, win, nperseg)
+    m1 = nperseg // 2 + 1
+    rows = x2.shape[0]
+    pxx = torch.empty(rows, m1, device=x2.device, dtype=torch.float32)
+    pyy = torch.empty_like(pxx)
+    pxy = torch.empty(rows, m1, device=x2.device, dtype=torch.complex64)
+    ws, wsb = _ws(_hip.COH_TRIO_MEAN, rows, n, nperseg, hop, x2.device)
+    lib = _hip.load()
+    rc = lib.msig_coherence_mean(_hip.ptr(x2), _hip.ptr(y2), rows, n, _hip.ptr(win), nperseg,
+                                 hop, int(_is_const(detrend)), _hip.ptr(pxx), _hip.ptr(pyy),
+                                 _hip.ptr(pxy), _hip.ptr(ws), wsb, _hip.stream_of(x2))
+    _hip.check(rc, "coherence_mean")
+    launches["op_coherence_mean"] += 1
+    return pxx.reshape(*batch, m1), pyy.reshape(*batch, m1), pxy.reshape(*batch, m1)

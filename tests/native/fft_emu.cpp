@@ -1,0 +1,173 @@
+// Host emulation of the device FFT: runs the same msig_fft.hpp templates the
+// kernels use, thread by thread with an emulated LDS array, in the same
+// store / barrier / load order, and checks
+//   1. the FFT against an O(N^2) FP64 DFT,
+//   2. that every LDS slot of a pass is written exactly once and every read
+//      slot was written (no stale or clobbered data),
+//   3. the two-for-one separation and mirror fold used by the kernels:
+//      |X_s|^2 + |X_{s+1}|^2 == (|Z[k]|^2 + |Z[N-k]|^2) / 2, and
+//      conj(X_s)Y_s + conj(X_{s+1})Y_{s+1} == (Q[k] + conj Q[N-k]) / 2
+//      with Q = conj(Zx) Zy.
+// Prints one line per size and exits non-zero on any failure.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "msig_fft.hpp"
+
+using msig::cf;
+using cd = std::complex<double>;
+
+static std::vector<cd> dft(const std::vector<cd>& z) {
+  const int N = (int)z.size();
+  std::vector<cd> out(N);
+  for (int k = 0; k < N; ++k) {
+    cd s = 0;
+    for (int n = 0; n < N; ++n) s += z[n] * std::polar(1.0, -2.0 * M_PI * double((int64_t)n * k % N) / N);
+    out[k] = s;
+  }
+  return out;
+}
+
+template <int N, int E, int P>
+static bool run_pass(std::vector<std::vector<cf>>& v, const std::vector<std::vector<cf>>& tw,
+                     std::vector<cf>& lds, std::vector<int>& wcount) {
+  using PL = msig::Plan<N, E>;
+  constexpr int TPF = PL::TPF;
+  if constexpr (P < PL::NP) {
+    std::fill(wcount.begin(), wcount.end(), 0);
+    for (int t = 0; t < TPF; ++t) {
+      msig::pass_store<N, E, P - 1>(t, v[t].data(), lds.data());
+      for (int e = 0; e < E; ++e) wcount[PL::pad(PL::template out_pos<P - 1>(t, e))]++;
+    }
+    for (int i = 0; i < (int)wcount.size(); ++i)
+      if (wcount[i] > 1) {
+        std::printf("N=%d pass %d: LDS slot %d written %d times\n", N, P - 1, i, wcount[i]);
+        return false;
+      }
+    for (int t = 0; t < TPF; ++t) {
+      for (int e = 0; e < E; ++e) {
+        const int slot = PL::pad(PL::template in_pos<P>(t, e));
+        if (wcount[slot] != 1) {
+          std::printf("N=%d pass %d: read of unwritten slot %d\n", N, P, slot);
+          return false;
+        }
+      }
+      msig::pass_load<N, E, P>(t, v[t].data(), lds.data());
+    }
+    for (int t = 0; t < TPF; ++t) msig::pass_compute<N, E, P>(v[t].data(), tw[t].data());
+    return run_pass<N, E, P + 1>(v, tw, lds, wcount);
+  }
+  return true;
+}
+
+// FFT of z (N complex points) exactly as a group of TPF device threads does it.
+template <int N, int E>
+static bool emu_fft(const std::vector<cf>& table, const std::vector<cf>& z, std::vector<cf>& Z) {
+  using PL = msig::Plan<N, E>;
+  constexpr int TPF = PL::TPF;
+  std::vector<std::vector<cf>> v(TPF, std::vector<cf>(E)), tw(TPF, std::vector<cf>(PL::TWN));
+  for (int t = 0; t < TPF; ++t) {
+    msig::load_twiddles<N, E>(t, table.data(), tw[t].data());
+    for (int e = 0; e < E; ++e) v[t][e] = z[t + TPF * e];
+    msig::pass_compute<N, E, 0>(v[t].data(), tw[t].data());
+  }
+  std::vector<cf> lds(PL::LDS);
+  std::vector<int> wcount(PL::LDS);
+  if (!run_pass<N, E, 1>(v, tw, lds, wcount)) return false;
+  Z.assign(N, cf{NAN, NAN});
+  std::vector<int> seen(N, 0);
+  for (int t = 0; t < TPF; ++t)
+    for (int e = 0; e < E; ++e) {
+      const int k = PL::bin(t, e);
+      if (k < 0 || k >= N || seen[k]) {
+        std::printf("N=%d: bad final bin map t=%d e=%d k=%d\n", N, t, e, k);
+        return false;
+      }
+      seen[k] = 1;
+      Z[k] = v[t][e];
+    }
+  return true;
+}
+
+template <int N>
+static int check_size(std::mt19937_64& rng) {
+  constexpr int E = N < 16 ? N : 16;
+  std::vector<cf> table(N);
+  for (int i = 0; i < N; ++i) {
+    const double a = -2.0 * M_PI * i / N;
+    table[i] = cf{(float)std::cos(a), (float)std::sin(a)};
+  }
+  std::normal_distribution<double> nd;
+  // 1. FFT vs DFT on random complex input
+  std::vector<cf> z(N), Z;
+  std::vector<cd> zd(N);
+  for (int i = 0; i < N; ++i) {
+    z[i] = cf{(float)nd(rng), (float)nd(rng)};
+    zd[i] = cd(z[i].re, z[i].im);
+  }
+  if (!emu_fft<N, E>(table, z, Z)) return 1;
+  auto ref = dft(zd);
+  double peak = 0, err = 0;
+  for (int k = 0; k < N; ++k) {
+    peak = std::max(peak, std::abs(ref[k]));
+    err = std::max(err, std::abs(cd(Z[k].re, Z[k].im) - ref[k]));
+  }
+  const double rel = err / peak;
+  // 2. two-for-one identities on real segment pairs / signal pairs
+  std::vector<cf> za(N), zb(N), Za, Zb;
+  std::vector<cd> xa(N), xb(N), ya(N), yb(N);
+  for (int i = 0; i < N; ++i) {
+    xa[i] = nd(rng);
+    xb[i] = nd(rng);
+    ya[i] = 0.5 * xa[i].real() + nd(rng);
+    yb[i] = 0.5 * xb[i].real() + nd(rng);
+    za[i] = cf{(float)xa[i].real(), (float)xb[i].real()};
+    zb[i] = cf{(float)ya[i].real(), (float)yb[i].real()};
+  }
+  if (!emu_fft<N, E>(table, za, Za) || !emu_fft<N, E>(table, zb, Zb)) return 1;
+  auto Xa = dft(xa), Xb = dft(xb), Ya = dft(ya), Yb = dft(yb);
+  double ppk = 0, perr = 0, cpk = 0, cerr = 0;
+  for (int k = 0; k < N; ++k) {
+    const int mk = (N - k) & (N - 1);
+    const double s = 0.5 * (std::norm(cd(Za[k].re, Za[k].im)) + std::norm(cd(Za[mk].re, Za[mk].im)));
+    const double sref = std::norm(Xa[k]) + std::norm(Xb[k]);
+    ppk = std::max(ppk, sref);
+    perr = std::max(perr, std::abs(s - sref));
+    const cd q = std::conj(cd(Za[k].re, Za[k].im)) * cd(Zb[k].re, Zb[k].im);
+    const cd qm = std::conj(cd(Za[mk].re, Za[mk].im)) * cd(Zb[mk].re, Zb[mk].im);
+    const cd c = 0.5 * (q + std::conj(qm));
+    const cd cref = std::conj(Xa[k]) * Ya[k] + std::conj(Xb[k]) * Yb[k];
+    cpk = std::max(cpk, std::abs(cref));
+    cerr = std::max(cerr, std::abs(c - cref));
+    // per-frame separation used by the frames kernel
+    const cd a(Za[k].re, Za[k].im), b(Za[mk].re, Za[mk].im);
+    const cd x0 = 0.5 * (a + std::conj(b));
+    const cd x1 = cd(0, -0.5) * (a - std::conj(b));
+    if (std::abs(x0 - Xa[k]) > 1e-4 * std::sqrt(ppk) || std::abs(x1 - Xb[k]) > 1e-4 * std::sqrt(ppk)) {
+      std::printf("N=%d: separation mismatch at k=%d\n", N, k);
+      return 1;
+    }
+  }
+  const bool ok = rel < 2e-6 && perr / ppk < 2e-6 && cerr / cpk < 2e-6;
+  std::printf("N=%5d E=%2d passes=%d fft_rel=%.3e psd_fold_rel=%.3e csd_fold_rel=%.3e %s\n", N, E,
+              msig::Plan<N, E>::NP, rel, perr / ppk, cerr / cpk, ok ? "ok" : "FAIL");
+  return ok ? 0 : 1;
+}
+
+int main() {
+  std::mt19937_64 rng(1234);
+  int bad = 0;
+  bad += check_size<16>(rng);
+  bad += check_size<32>(rng);
+  bad += check_size<64>(rng);
+  bad += check_size<128>(rng);
+  bad += check_size<256>(rng);
+  bad += check_size<512>(rng);
+  bad += check_size<1024>(rng);
+  bad += check_size<2048>(rng);
+  bad += check_size<4096>(rng);
+  return bad ? 1 : 0;
+}

@@ -1,0 +1,62 @@
+"""The C ABI: libmsig.so loads on a machine without a GPU and exports every
+entry point include/msig.h declares (no compute calls here)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from mlx_signal_amd import _hip
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared():
+    text = (ROOT / "include" / "msig.h").read_text()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(msig_\w+)\s*\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert _declared() == sorted(_hip.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    if not _hip.LIB_PATH.exists():
+        pytest.skip("libmsig.so not built")
+    lib = ctypes.CDLL(str(_hip.LIB_PATH))
+    for name in _declared():
+        assert hasattr(lib, name), name
+    b = _hip.load()
+    assert b.msig_abi_version() == _hip.ABI_VERSION
+    assert b.msig_device_count() >= 0
+
+
+def test_eligibility_predicate():
+    if not _hip.LIB_PATH.exists():
+        pytest.skip("libmsig.so not built")
+    lib = _hip.load()
+    for n in (16, 64, 256, 1024, 4096):
+        assert lib.msig_eligible(n, n, 1, 1, 0) == 1
+        assert lib.msig_eligible(n // 2 + 1, n, 1, 0, 0) == 1  # zero-padded segments
+    assert lib.msig_eligible(100, 100, 1, 1, 0) == 0   # non power of two
+    assert lib.msig_eligible(8192, 8192, 1, 1, 0) == 0  # beyond the fused range
+    assert lib.msig_eligible(256, 128, 1, 1, 0) == 0   # nfft < nperseg
+    assert lib.msig_eligible(8, 8, 1, 1, 0) == 0
+
+
+def test_bad_descriptor_rejected_without_gpu():
+    if not _hip.LIB_PATH.exists():
+        pytest.skip("libmsig.so not built")
+    lib = _hip.load()
+    job = _hip.MsigJob(_hip.PSD_MEAN, _hip.F_ONESIDED, 4, 1000, 1000, 1000, 256, 128, 256, 99, 1.0)
+    sz = ctypes.c_size_t(0)
+    assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == _hip.MSIG_EINVAL
+    assert b"nseg" in lib.msig_last_error()
+    job.nseg = (1000 - 256) // 128 + 1
+    assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == 0
+    assert sz.value > 0
+    job.nfft = 100
+    job.nperseg = 100
+    job.nseg = (1000 - 100) // 128 + 1
+    assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == _hip.MSIG_ENOTSUP
