@@ -129,6 +129,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-ref", action="store_true",
+                    help="also run 3 device copies of a 256 MiB buffer (a known-size kernel "
+                         "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -163,6 +166,14 @@ def main():
     xr = xs[0][:2].cpu().numpy()
     ref = oracle.welch(xr, fs=FS, window="hann", nperseg=NPERSEG, noverlap=NOVERLAP)[1]
     err = float(np.max(np.abs(p[:2].cpu().numpy() - ref)) / np.max(np.abs(ref)))
+
+    if args.copy_ref:
+        src = xs[0]
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del dst
 
     secondary = {}
     if not args.no_secondary:

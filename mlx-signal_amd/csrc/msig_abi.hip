@@ -64,7 +64,7 @@ static int twiddles(int N, const cf** out) {
   if (e != hipSuccess) return hip_fail(e, "hipMalloc(twiddles)");
   e = hipMemcpy(d, h.data(), sizeof(cf) * N, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    hipFree(d);
+    (void)hipFree(d);
     return hip_fail(e, "hipMemcpy(twiddles)");
   }
   g_tw[{dev, N}] = d;
@@ -72,79 +72,21 @@ static int twiddles(int N, const cf** out) {
   return 0;
 }
 
-// ---------------------------------------------------------------- finalize
-struct FinArgs {
-  const float* ws;
-  void* out0;
-  void* out1;
-  void* out2;
-  int64_t rows;
-  int32_t bpr, N, nbins, nc, kind, fold, apply_factor;
-  double scale;  // scale / nseg
-};
-
-// Sum the block partials of each row in FP64 (fixed order), fold the
-// conjugate mirror of the two-for-one packing, apply scale, 1/nseg and the
-// onesided factor.  Counterpart of _reduce_groups (_stft_metal.py:443-455)
-// plus the deferred factor of csd (spectral.py:482-488).
-__global__ __launch_bounds__(256) void k_finalize(const FinArgs f) {
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= f.rows * f.nbins) return;
-  const int64_t row = idx / f.nbins;
-  const int k = (int)(idx - row * f.nbins);
-  const int N = f.N;
-  const int mk = (N - k) & (N - 1);
-  const float* base = f.ws + (size_t)row * f.bpr * f.nc * N;
-  double s[4] = {0, 0, 0, 0}, m[4] = {0, 0, 0, 0};
-  for (int b = 0; b < f.bpr; ++b) {
-    const float* pb = base + (size_t)b * f.nc * N;
-    for (int c = 0; c < f.nc; ++c) {
-      s[c] += (double)pb[c * N + k];
-      if (f.fold) m[c] += (double)pb[c * N + mk];
-    }
-  }
-  double p[4];
-  if (f.fold) {
-    // real parts / powers fold symmetric, imaginary parts antisymmetric
-    if (f.nc == 1) {
-      p[0] = 0.5 * (s[0] + m[0]);
-    } else if (f.nc == 2) {
-      p[0] = 0.5 * (s[0] + m[0]);
-      p[1] = 0.5 * (s[1] - m[1]);
-    } else {
-      p[0] = 0.5 * (s[0] + m[0]);
-      p[1] = 0.5 * (s[1] + m[1]);
-      p[2] = 0.5 * (s[2] + m[2]);
-      p[3] = 0.5 * (s[3] - m[3]);
-    }
-  } else {
-    for (int c = 0; c < 4; ++c) p[c] = s[c];
-  }
-  double sc = f.scale;
-  if (f.apply_factor && !(k == 0 || 2 * k == N)) sc *= 2.0;
-  if (f.kind == MSIG_PSD_MEAN) {
-    static_cast<float*>(f.out0)[idx] = (float)(p[0] * sc);
-  } else if (f.kind == MSIG_CSD_MEAN) {
-    static_cast<cf*>(f.out0)[idx] = cf{(float)(p[0] * sc), (float)(p[1] * sc)};
-  } else if (f.kind == MSIG_COH_TRIO_MEAN) {
-    static_cast<float*>(f.out0)[idx] = (float)(p[0] * sc);
-    static_cast<float*>(f.out1)[idx] = (float)(p[1] * sc);
-    static_cast<cf*>(f.out2)[idx] = cf{(float)(p[2] * sc), (float)(p[3] * sc)};
-  } else {  // MSIG_COHERENCE: scales cancel in the ratio
-    const double num = p[2] * p[2] + p[3] * p[3];
-    static_cast<float*>(f.out0)[idx] = (float)(num / p[0] / p[1]);
-  }
-}
-
 // ---------------------------------------------------------------- dispatch
 static bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// hop == N/2 with full-length segments: each sample is loaded once per group
+static bool half_hop(const KArgs& a, int N, bool cplx) {
+  return !cplx && a.nperseg == N && 2 * a.hop == N;
+}
+
 static hipError_t dispatch_mean(int N, int kk, bool cplx, const KArgs& a, int64_t rows,
                                 hipStream_t s) {
+  const bool half = half_hop(a, N, cplx);
   switch (N) {
 #define C(NV) \
   case NV:    \
-    return launch_mean_##NV(kk, cplx, a, rows, s);
+    return launch_mean_##NV(kk, cplx, half, a, rows, s);
     C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
   }
@@ -153,10 +95,11 @@ static hipError_t dispatch_mean(int N, int kk, bool cplx, const KArgs& a, int64_
 
 static hipError_t dispatch_frames(int N, int ok, bool cplx, const KArgs& a, int64_t rows,
                                   hipStream_t s) {
+  const bool half = half_hop(a, N, cplx);
   switch (N) {
 #define C(NV) \
   case NV:    \
-    return launch_frames_##NV(ok, cplx, a, rows, s);
+    return launch_frames_##NV(ok, cplx, half, a, rows, s);
     C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
   }
@@ -210,10 +153,14 @@ static int nbins_of(const msig_job* j) {
   return (!cplx && (j->flags & MSIG_F_ONESIDED)) ? j->nfft / 2 + 1 : j->nfft;
 }
 
+// Mean-kind workspace: per-row tickets (int32, padded to 256 bytes) followed
+// by the block partials [rows][bpr][ncomp][nfft] float32.
+static size_t ticket_bytes(int64_t rows) { return ((size_t)rows * sizeof(int32_t) + 255) / 256 * 256; }
+
 static size_t mean_ws_bytes(const msig_job* j) {
   const bool cplx = j->flags & MSIG_F_COMPLEX_INPUT;
   const Geo g = geometry(j->rows, j->nseg, j->nfft, cplx, MEAN_MAX_RUN);
-  return (size_t)j->rows * g.bpr * ncomp(j->kind) * j->nfft * sizeof(float);
+  return ticket_bytes(j->rows) + (size_t)j->rows * g.bpr * ncomp(j->kind) * j->nfft * sizeof(float);
 }
 
 static KArgs base_args(const msig_job* j, const void* x, const void* y, const float* win,
@@ -303,30 +250,20 @@ int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const 
   rc = twiddles(N, &tw);
   if (rc) return rc;
   KArgs a = base_args(job, x, y, win, tw, g);
-  a.ws = static_cast<float*>(workspace);
+  a.tickets = static_cast<int32_t*>(workspace);
+  a.ws = reinterpret_cast<float*>(static_cast<char*>(workspace) + ticket_bytes(job->rows));
+  a.out0 = out0;
+  a.out1 = out1;
+  a.out2 = out2;
+  a.nbins = nbins_of(job);
+  a.out_kind = job->kind;
+  a.fold = cplx ? 0 : 1;
+  a.apply_factor =
+      (!cplx && (job->flags & MSIG_F_APPLY_FACTOR) && (job->flags & MSIG_F_ONESIDED)) ? 1 : 0;
+  a.mscale = job->scale / (double)job->nseg;
   const int kk = job->kind == MSIG_PSD_MEAN ? K_PSD : (job->kind == MSIG_CSD_MEAN ? K_CSD : K_COH);
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  hipError_t e = dispatch_mean(N, kk, cplx, a, job->rows, s);
+  hipError_t e = dispatch_mean(N, kk, cplx, a, job->rows, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "fused mean kernel launch");
-  FinArgs f;
-  f.ws = a.ws;
-  f.out0 = out0;
-  f.out1 = out1;
-  f.out2 = out2;
-  f.rows = job->rows;
-  f.bpr = g.bpr;
-  f.N = N;
-  f.nbins = nbins_of(job);
-  f.nc = ncomp(job->kind);
-  f.kind = job->kind;
-  f.fold = cplx ? 0 : 1;
-  f.apply_factor = (!cplx && (job->flags & MSIG_F_APPLY_FACTOR) && (job->flags & MSIG_F_ONESIDED)) ? 1 : 0;
-  f.scale = job->scale / (double)job->nseg;
-  const int64_t total = job->rows * f.nbins;
-  const dim3 grid((unsigned)((total + 255) / 256)), block(256);
-  hipLaunchKernelGGL(k_finalize, grid, block, 0, s, f);
-  e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(e, "finalize kernel launch");
   return 0;
 }
 

@@ -223,8 +223,11 @@ MSIG_HD void load_twiddles(int t, const cf* table, cf* tw) {
   }
 }
 
-// Pass P compute step on registers already read for that pass.
-template <int N, int E, int P>
+// Pass P compute step on registers already read for that pass.  Twiddle
+// slot s of this thread is tw[s * TS]: TS = 1 for a per-thread register
+// array, TS = TPF for the slot-major LDS table (one column per thread
+// position, so every read is one ds_read_b64 at a constant offset).
+template <int N, int E, int P, int TS = 1>
 MSIG_HD void pass_compute(cf* v, const cf* tw) {
   using PL = Plan<N, E>;
   constexpr int R = PL::radix(P);
@@ -233,7 +236,8 @@ MSIG_HD void pass_compute(cf* v, const cf* tw) {
     if constexpr (P > 0) {
       constexpr int off = PL::tw_off(P);
 #pragma unroll
-      for (int j = 1; j < R; ++j) v[u * R + j] = cmul(v[u * R + j], tw[off + u * (R - 1) + (j - 1)]);
+      for (int j = 1; j < R; ++j)
+        v[u * R + j] = cmul(v[u * R + j], tw[(off + u * (R - 1) + (j - 1)) * TS]);
     }
     Dft<R>::run(v + u * R);
   }

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include "msig_fft.hpp"
+#include "msig_stream.hpp"
 
 namespace msig {
 
@@ -28,24 +29,6 @@ constexpr int BT = 256;  // threads per block
 enum : int { K_PSD = 0, K_CSD = 1, K_COH = 2 };
 enum : int { F_COMPLEX = 0, F_POWER = 1, F_CSD = 2, F_COH = 3 };
 
-struct KArgs {
-  const void* x;
-  const void* y;
-  const float* win;
-  const cf* twt;     // exp(-2 pi i e / N), e < N
-  float* ws;         // mean kinds: [rows][bpr][ncomp][N] block partials
-  void* out0;        // frames kinds
-  void* out1;
-  void* out2;
-  int64_t ld_x, ld_y;
-  int64_t nseg;
-  int32_t nperseg, hop;
-  int32_t units, run, bpr;
-  int32_t detrend;
-  int32_t nbins;     // frames: bins written per frame
-  int32_t apply_factor;
-  float oscale;      // frames: power scale or sqrt(scale) for complex output
-};
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -97,61 +80,53 @@ __device__ __forceinline__ void group_sum2(float& a, float& b, float* red) {
   }
 }
 
-// Forward FFT of the group's N points held E per thread.
-template <int N, int E, int P = 1>
+// Where a kernel keeps its Stockham twiddles: in an LDS table (slot-major,
+// TWN x TPF) when that table is small, else in registers.  The LDS table
+// frees ~2*TWN VGPRs per thread, which buys occupancy.
+template <int N, int E>
+struct TwiddleHome {
+  using PL = Plan<N, E>;
+  static constexpr bool LDS = (PL::NP > 1) && (PL::TWN * PL::TPF * 8 <= 32768);
+  static constexpr int LDS_ELEMS = LDS ? PL::TWN * PL::TPF : 1;
+  static constexpr int REGS = LDS ? 1 : PL::TWN;
+  static constexpr int TS = LDS ? PL::TPF : 1;
+};
+
+// Fill the slot-major LDS twiddle table from the global FP64-derived table.
+template <int N, int E>
+__device__ __forceinline__ void fill_twiddle_lds(cf* twl, const cf* table) {
+  using PL = Plan<N, E>;
+  for (int t = threadIdx.x; t < PL::TPF; t += BT) {
+    cf tmp[PL::TWN];
+    load_twiddles<N, E>(t, table, tmp);
+#pragma unroll
+    for (int s = 0; s < PL::TWN; ++s) twl[s * PL::TPF + t] = tmp[s];
+  }
+}
+
+// Forward FFT of the group's N points held E per thread; tw points at this
+// thread's twiddle slot 0 (stride TS between slots).
+template <int N, int E, int TS, int P = 1>
 __device__ __forceinline__ void fft_passes(int t, cf* v, const cf* tw, cf* lds) {
   using PL = Plan<N, E>;
-  if constexpr (P == 1) pass_compute<N, E, 0>(v, tw);
+  if constexpr (P == 1) pass_compute<N, E, 0, TS>(v, tw);
   if constexpr (P < PL::NP) {
     pass_store<N, E, P - 1>(t, v, lds);
     group_sync<PL::TPF>();
     pass_load<N, E, P>(t, v, lds);
     if constexpr (PL::TPF > 64) __syncthreads();  // reads done before next stores
-    pass_compute<N, E, P>(v, tw);
-    fft_passes<N, E, P + 1>(t, v, tw, lds);
+    pass_compute<N, E, P, TS>(v, tw);
+    fft_passes<N, E, TS, P + 1>(t, v, tw, lds);
   }
 }
 
-// Load one unit (two real segments, or one complex segment) of a row,
-// detrend, window.  Units past the row's end load zeros.
-template <int N, int E, bool CPLX>
-__device__ __forceinline__ void load_unit(const void* xrow_v, int t, int unit, bool active,
-                                          const KArgs& a, const float* w, float* red, cf* z) {
-  using PL = Plan<N, E>;
-  constexpr int TPF = PL::TPF;
-  const bool full = (a.nperseg == N);
-  if constexpr (!CPLX) {
-    const float* xrow = static_cast<const float*>(xrow_v);
-    const int64_t s0 = 2 * (int64_t)unit;
-    const bool v0 = active;
-    const bool v1 = active && (s0 + 1 < a.nseg);
-    const float* p0 = xrow + s0 * a.hop;
-    const float* p1 = p0 + a.hop;
-    if (full && v1) {
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        z[e].re = p0[t + TPF * e];
-        z[e].im = p1[t + TPF * e];
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int n = t + TPF * e;
-        const bool in = n < a.nperseg;
-        z[e].re = (v0 && in) ? p0[n] : 0.f;
-        z[e].im = (v1 && in) ? p1[n] : 0.f;
-      }
-    }
-  } else {
-    const cf* xrow = static_cast<const cf*>(xrow_v);
-    const cf* p0 = xrow + (int64_t)unit * a.hop;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int n = t + TPF * e;
-      z[e] = (active && n < a.nperseg) ? p0[n] : cf{0.f, 0.f};
-    }
-  }
-  if (a.detrend) {
+// Detrend (segment mean over nperseg samples, per real/imag lane of the
+// packed pair) and window, in place.  Samples n >= nperseg carry w = 0.
+template <int N, int E>
+__device__ __forceinline__ void prep_unit(cf* z, const float* w, int detrend, int nperseg,
+                                          float* red) {
+  constexpr int TPF = N / E;
+  if (detrend) {
     float sr = 0.f, si = 0.f;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -159,7 +134,7 @@ __device__ __forceinline__ void load_unit(const void* xrow_v, int t, int unit, b
       si += z[e].im;
     }
     group_sum2<TPF>(sr, si, red);
-    const float inv = 1.0f / (float)a.nperseg;
+    const float inv = 1.0f / (float)nperseg;
     const float mr = sr * inv, mi = si * inv;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -175,16 +150,81 @@ __device__ __forceinline__ void load_unit(const void* xrow_v, int t, int unit, b
   }
 }
 
+// FP64 sum of a row's block partials in block order, conjugate-mirror fold
+// of the two-for-one packing, scale / nseg and the onesided factor.
+// Counterpart of _reduce_groups (_stft_metal.py:443-455) and the deferred
+// onesided factor of csd (spectral.py:482-488).
+template <int NC>
+__device__ __forceinline__ void finalize_row(const KArgs& a, int N, int64_t row, int tid) {
+  const float* base = a.ws + (size_t)row * a.bpr * NC * N;
+  const size_t bstride = (size_t)NC * N;
+  for (int k = tid; k < a.nbins; k += BT) {
+    const int mk = (N - k) & (N - 1);
+    double s[NC], m[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      s[c] = 0.0;
+      m[c] = 0.0;
+    }
+    int b = 0;
+    for (; b + 4 <= a.bpr; b += 4) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        float v[4], w[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float* pb = base + (b + i) * bstride + (size_t)c * N;
+          v[i] = __builtin_nontemporal_load(pb + k);
+          w[i] = a.fold ? __builtin_nontemporal_load(pb + mk) : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          s[c] += (double)v[i];
+          m[c] += (double)w[i];
+        }
+      }
+    }
+    for (; b < a.bpr; ++b) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const float* pb = base + b * bstride + (size_t)c * N;
+        s[c] += (double)__builtin_nontemporal_load(pb + k);
+        if (a.fold) m[c] += (double)__builtin_nontemporal_load(pb + mk);
+      }
+    }
+    double p[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      // powers and real parts fold symmetric, imaginary parts antisymmetric
+      const bool imag = (NC == 2 && c == 1) || (NC == 4 && c == 3);
+      p[c] = a.fold ? 0.5 * (imag ? s[c] - m[c] : s[c] + m[c]) : s[c];
+    }
+    double sc = a.mscale;
+    if (a.apply_factor && !(k == 0 || 2 * k == N)) sc *= 2.0;
+    const size_t o = (size_t)row * a.nbins + k;
+    if (a.out_kind == 0) {  // MSIG_PSD_MEAN
+      static_cast<float*>(a.out0)[o] = (float)(p[0] * sc);
+    } else if (a.out_kind == 1) {  // MSIG_CSD_MEAN
+      static_cast<cf*>(a.out0)[o] = cf{(float)(p[0] * sc), (float)(p[1] * sc)};
+    } else if (a.out_kind == 2) {  // MSIG_COH_TRIO_MEAN
+      static_cast<float*>(a.out0)[o] = (float)(p[0] * sc);
+      static_cast<float*>(a.out1)[o] = (float)(p[1] * sc);
+      static_cast<cf*>(a.out2)[o] = cf{(float)(p[2] * sc), (float)(p[3] * sc)};
+    } else {  // MSIG_COHERENCE: scale and factor cancel in the ratio
+      const double num = p[2] * p[2] + p[3] * p[3];
+      static_cast<float*>(a.out0)[o] = (float)(num / p[0] / p[1]);
+    }
+  }
+}
+
 // Grouped mean over segments: per-bin sums of |Zx|^2 (PSD), conj(Zx) Zy
 // (CSD) or both spectra and the cross term (COH), written per block.
-template <int N, int E, int KIND, bool CPLX>
-__global__ __launch_bounds__(BT) void k_mean(const KArgs a) {
+template <int N, int E, int KIND, bool CPLX, bool HALF>
+__device__ __forceinline__ void mean_body(const KArgs& a, cf* lds_all, float* red, const cf* twl) {
   using PL = Plan<N, E>;
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
   constexpr int NC = KIND == K_PSD ? 1 : (KIND == K_CSD ? 2 : 4);
-  __shared__ cf lds_all[GPB * PL::LDS];
-  __shared__ float red[2 * (BT / 64)];
   const int tid = threadIdx.x;
   const int g = tid / TPF, t = tid % TPF;
   cf* lds = lds_all + g * PL::LDS;
@@ -192,8 +232,15 @@ __global__ __launch_bounds__(BT) void k_mean(const KArgs a) {
   const int bir = blockIdx.x % a.bpr;
   const int unit0 = (bir * GPB + g) * a.run;
 
-  cf tw[PL::TWN];
-  load_twiddles<N, E>(t, a.twt, tw);
+  using TH = TwiddleHome<N, E>;
+  cf twr[TH::REGS];
+  const cf* tw;
+  if constexpr (TH::LDS) {
+    tw = twl + t;
+  } else {
+    load_twiddles<N, E>(t, a.twt, twr);
+    tw = twr;
+  }
   float w[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -201,8 +248,13 @@ __global__ __launch_bounds__(BT) void k_mean(const KArgs a) {
     w[e] = n < a.nperseg ? a.win[n] : 0.f;
   }
   const size_t esz = CPLX ? sizeof(cf) : sizeof(float);
-  const char* xrow = static_cast<const char*>(a.x) + (size_t)row * a.ld_x * esz;
-  const char* yrow = KIND == K_PSD ? nullptr : static_cast<const char*>(a.y) + (size_t)row * a.ld_y * esz;
+  UnitStream<N, E, CPLX, HALF> sx, sy;
+  sx.row = static_cast<const char*>(a.x) + (size_t)row * a.ld_x * esz;
+  sx.t = t;
+  if constexpr (KIND != K_PSD) {
+    sy.row = static_cast<const char*>(a.y) + (size_t)row * a.ld_y * esz;
+    sy.t = t;
+  }
 
   float acc[NC][E];
 #pragma unroll
@@ -210,22 +262,34 @@ __global__ __launch_bounds__(BT) void k_mean(const KArgs a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[c][e] = 0.f;
 
+  sx.fetch(unit0, true, a);
+  if constexpr (KIND != K_PSD) sy.fetch(unit0, true, a);
   for (int it = 0; it < a.run; ++it) {
     const int unit = unit0 + it;
     const bool active = unit < a.units;
+    const bool more = it + 1 < a.run;
     cf zx[E];
-    load_unit<N, E, CPLX>(xrow, t, unit, active, a, w, red, zx);
-    fft_passes<N, E>(t, zx, tw, lds);
+    sx.take(zx);
     if constexpr (KIND == K_PSD) {
+      if (more) sx.fetch(unit + 1, false, a);
+      prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
+      fft_passes<N, E, TH::TS>(t, zx, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) acc[0][e] += zx[e].re * zx[e].re + zx[e].im * zx[e].im;
       }
     } else {
       cf zy[E];
+      sy.take(zy);
+      if (more) {
+        sx.fetch(unit + 1, false, a);
+        sy.fetch(unit + 1, false, a);
+      }
+      prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
+      fft_passes<N, E, TH::TS>(t, zx, tw, lds);
       group_sync<TPF>();  // last-pass reads of zx done before zy's stores
-      load_unit<N, E, CPLX>(yrow, t, unit, active, a, w, red, zy);
-      fft_passes<N, E>(t, zy, tw, lds);
+      prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
+      fft_passes<N, E, TH::TS>(t, zy, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -261,18 +325,77 @@ __global__ __launch_bounds__(BT) void k_mean(const KArgs a) {
       dst[c * N + k] = s;
     }
   }
+
+  // The last block of a row to finish reduces the row's partials
+  // (threadFence-reduction pattern): publish, take a ticket, and if it is
+  // the row's final ticket, read every block's partial back from L2.
+  __threadfence();
+  __syncthreads();
+  __shared__ int is_last;
+  if (tid == 0) is_last = (atomicAdd(a.tickets + row, 1) == a.bpr - 1);
+  __syncthreads();
+  if (is_last) {
+    __threadfence();
+    finalize_row<NC>(a, N, row, tid);
+    if (tid == 0) a.tickets[row] = 0;  // leave the ticket zeroed for the next call
+  }
+}
+
+// Waves per SIMD each kernel is compiled for.  Single-signal kinds fit three
+// waves without spills; two-signal kinds hold both spectra and their
+// prefetch buffers, and run with two (CSD) or as many as fit (coherence).
+template <int N, int KIND>
+constexpr int mean_waves() {
+  return N >= 4096 ? (KIND == K_PSD ? 2 : 1)
+                   : N >= 2048 ? (KIND == K_PSD ? 2 : 1) : (KIND == K_PSD ? 3 : (KIND == K_CSD ? 2 : 1));
+}
+template <int N, int OK>
+constexpr int frames_waves() {
+  return N >= 2048 ? ((OK == F_COMPLEX || OK == F_POWER) ? 2 : 1)
+                   : ((OK == F_COMPLEX || OK == F_POWER) ? 3 : 2);
+}
+
+// HALF selects the hop == N/2 register-reuse stream (real input, nperseg == N).
+template <int N, int E, int KIND, bool CPLX, bool HALF>
+__global__ __launch_bounds__(BT, (mean_waves<N, KIND>())) void k_mean(const KArgs a) {
+  using PL = Plan<N, E>;
+  __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
+  __shared__ float red[2 * (BT / 64)];
+  __shared__ cf twl[TwiddleHome<N, E>::LDS_ELEMS];
+  if constexpr (TwiddleHome<N, E>::LDS) {
+    fill_twiddle_lds<N, E>(twl, a.twt);
+    __syncthreads();
+  }
+  mean_body<N, E, KIND, CPLX, HALF>(a, lds_all, red, twl);
 }
 
 // Per-frame spectra.  Real input: the two packed segments are separated with
 // X_s = (Z[k] + conj Z[N-k]) / 2, X_{s+1} = -i (Z[k] - conj Z[N-k]) / 2.
-template <int N, int E, int OK, bool CPLX>
-__global__ __launch_bounds__(BT) void k_frames(const KArgs a) {
+template <int N, int E>
+__device__ __forceinline__ void separate(int t, const cf* z, cf* lds, cf* x0, cf* x1) {
+  using PL = Plan<N, E>;
+  constexpr int TPF = PL::TPF;
+  group_sync<TPF>();
+#pragma unroll
+  for (int e = 0; e < E; ++e) lds[PL::pad(PL::bin(t, e))] = z[e];
+  group_sync<TPF>();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + TPF * e;
+    const cf p = lds[PL::pad(k)];
+    const cf q = lds[PL::pad((N - k) & (N - 1))];
+    x0[e] = cf{0.5f * (p.re + q.re), 0.5f * (p.im - q.im)};
+    x1[e] = cf{0.5f * (p.im + q.im), -0.5f * (p.re - q.re)};
+  }
+  group_sync<TPF>();
+}
+
+template <int N, int E, int OK, bool CPLX, bool HALF>
+__device__ __forceinline__ void frames_body(const KArgs& a, cf* lds_all, float* red, const cf* twl) {
   using PL = Plan<N, E>;
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
   constexpr bool PAIR = (OK == F_CSD || OK == F_COH);
-  __shared__ cf lds_all[GPB * PL::LDS];
-  __shared__ float red[2 * (BT / 64)];
   const int tid = threadIdx.x;
   const int g = tid / TPF, t = tid % TPF;
   cf* lds = lds_all + g * PL::LDS;
@@ -280,8 +403,15 @@ __global__ __launch_bounds__(BT) void k_frames(const KArgs a) {
   const int bir = blockIdx.x % a.bpr;
   const int unit0 = (bir * GPB + g) * a.run;
 
-  cf tw[PL::TWN];
-  load_twiddles<N, E>(t, a.twt, tw);
+  using TH = TwiddleHome<N, E>;
+  cf twr[TH::REGS];
+  const cf* tw;
+  if constexpr (TH::LDS) {
+    tw = twl + t;
+  } else {
+    load_twiddles<N, E>(t, a.twt, twr);
+    tw = twr;
+  }
   float w[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -289,24 +419,37 @@ __global__ __launch_bounds__(BT) void k_frames(const KArgs a) {
     w[e] = n < a.nperseg ? a.win[n] : 0.f;
   }
   const size_t esz = CPLX ? sizeof(cf) : sizeof(float);
-  const char* xrow = static_cast<const char*>(a.x) + (size_t)row * a.ld_x * esz;
-  const char* yrow = PAIR ? static_cast<const char*>(a.y) + (size_t)row * a.ld_y * esz : nullptr;
+  UnitStream<N, E, CPLX, HALF> sx, sy;
+  sx.row = static_cast<const char*>(a.x) + (size_t)row * a.ld_x * esz;
+  sx.t = t;
+  if constexpr (PAIR) {
+    sy.row = static_cast<const char*>(a.y) + (size_t)row * a.ld_y * esz;
+    sy.t = t;
+  }
   const int nb = a.nbins;
   const float os = a.oscale;
 
+  sx.fetch(unit0, true, a);
+  if constexpr (PAIR) sy.fetch(unit0, true, a);
   for (int it = 0; it < a.run; ++it) {
     const int unit = unit0 + it;
     const bool active = unit < a.units;
-    cf zx[E];
-    load_unit<N, E, CPLX>(xrow, t, unit, active, a, w, red, zx);
-    fft_passes<N, E>(t, zx, tw, lds);
+    const bool more = it + 1 < a.run;
+    cf zx[E], zy[E];
+    sx.take(zx);
+    if constexpr (PAIR) sy.take(zy);
+    if (more) {
+      sx.fetch(unit + 1, false, a);
+      if constexpr (PAIR) sy.fetch(unit + 1, false, a);
+    }
+    prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
+    fft_passes<N, E, TH::TS>(t, zx, tw, lds);
     if constexpr (CPLX) {
       // one segment per unit; bins come out in registers, no separation
-      cf zy[E];
       if constexpr (PAIR) {
         group_sync<TPF>();
-        load_unit<N, E, CPLX>(yrow, t, unit, active, a, w, red, zy);
-        fft_passes<N, E>(t, zy, tw, lds);
+        prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
+        fft_passes<N, E, TH::TS>(t, zy, tw, lds);
       }
       if (active) {
         const int64_t fr = row * a.nseg + unit;
@@ -330,38 +473,13 @@ __global__ __launch_bounds__(BT) void k_frames(const KArgs a) {
         }
       }
     } else {
-      // separate the two real segments through the conjugate mirror
       cf x0[E], x1[E];
-      group_sync<TPF>();
-#pragma unroll
-      for (int e = 0; e < E; ++e) lds[PL::pad(PL::bin(t, e))] = zx[e];
-      group_sync<TPF>();
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int k = t + TPF * e;
-        const cf p = lds[PL::pad(k)];
-        const cf q = lds[PL::pad((N - k) & (N - 1))];
-        x0[e] = cf{0.5f * (p.re + q.re), 0.5f * (p.im - q.im)};
-        x1[e] = cf{0.5f * (p.im + q.im), -0.5f * (p.re - q.re)};
-      }
+      separate<N, E>(t, zx, lds, x0, x1);
       cf y0[E], y1[E];
       if constexpr (PAIR) {
-        group_sync<TPF>();
-        cf zy[E];
-        load_unit<N, E, CPLX>(yrow, t, unit, active, a, w, red, zy);
-        fft_passes<N, E>(t, zy, tw, lds);
-        group_sync<TPF>();
-#pragma unroll
-        for (int e = 0; e < E; ++e) lds[PL::pad(PL::bin(t, e))] = zy[e];
-        group_sync<TPF>();
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const int k = t + TPF * e;
-          const cf p = lds[PL::pad(k)];
-          const cf q = lds[PL::pad((N - k) & (N - 1))];
-          y0[e] = cf{0.5f * (p.re + q.re), 0.5f * (p.im - q.im)};
-          y1[e] = cf{0.5f * (p.im + q.im), -0.5f * (p.re - q.re)};
-        }
+        prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
+        fft_passes<N, E, TH::TS>(t, zy, tw, lds);
+        separate<N, E>(t, zy, lds, y0, y1);
       }
       if (active) {
         const int64_t s0 = 2 * (int64_t)unit;
@@ -399,6 +517,19 @@ __global__ __launch_bounds__(BT) void k_frames(const KArgs a) {
     }
     group_sync<TPF>();
   }
+}
+
+template <int N, int E, int OK, bool CPLX, bool HALF>
+__global__ __launch_bounds__(BT, (frames_waves<N, OK>())) void k_frames(const KArgs a) {
+  using PL = Plan<N, E>;
+  __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
+  __shared__ float red[2 * (BT / 64)];
+  __shared__ cf twl[TwiddleHome<N, E>::LDS_ELEMS];
+  if constexpr (TwiddleHome<N, E>::LDS) {
+    fill_twiddle_lds<N, E>(twl, a.twt);
+    __syncthreads();
+  }
+  frames_body<N, E, OK, CPLX, HALF>(a, lds_all, red, twl);
 }
 
 }  // namespace msig

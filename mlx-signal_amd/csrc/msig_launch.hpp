@@ -41,9 +41,10 @@ constexpr int FRAMES_MAX_RUN = 8;
 
 // One pair of launchers per fused FFT size, defined in msig_n<N>.hip.
 #define MSIG_DECLARE_LAUNCHERS(NV)                                                         \
-  hipError_t launch_mean_##NV(int kind, bool cplx, const KArgs& a, int64_t rows,           \
+  hipError_t launch_mean_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows,  \
                               hipStream_t s);                                              \
-  hipError_t launch_frames_##NV(int ok, bool cplx, const KArgs& a, int64_t rows, hipStream_t s);
+  hipError_t launch_frames_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows, \
+                                hipStream_t s);
 MSIG_DECLARE_LAUNCHERS(16)
 MSIG_DECLARE_LAUNCHERS(32)
 MSIG_DECLARE_LAUNCHERS(64)
@@ -57,38 +58,43 @@ MSIG_DECLARE_LAUNCHERS(4096)
 }  // namespace msig
 
 // Defines launch_mean_<N> / launch_frames_<N> in the translation unit that
-// instantiates the kernels for FFT size N.
+// instantiates the kernels for FFT size N.  `half` selects the hop == N/2
+// register-reuse kernels (real input, nperseg == N).
+#define MSIG_LAUNCH_MEAN(NV, K, C, H) \
+  hipLaunchKernelGGL((k_mean<NV, (NV < 16 ? NV : 16), K, C, H>), grid, block, 0, s, a)
+#define MSIG_LAUNCH_FRAMES(NV, K, C, H) \
+  hipLaunchKernelGGL((k_frames<NV, (NV < 16 ? NV : 16), K, C, H>), grid, block, 0, s, a)
+#define MSIG_KIND_SWITCH(L, NV, K0, K1, K2, C, H) \
+  if (kind == K0) L(NV, K0, C, H);                \
+  else if (kind == K1) L(NV, K1, C, H);           \
+  else L(NV, K2, C, H);
+
 #define MSIG_DEFINE_LAUNCHERS(NV)                                                          \
   namespace msig {                                                                         \
-  hipError_t launch_mean_##NV(int kind, bool cplx, const KArgs& a, int64_t rows,            \
-                             hipStream_t s) {                                              \
-    constexpr int E = NV < 16 ? NV : 16;                                                   \
+  hipError_t launch_mean_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows, \
+                              hipStream_t s) {                                             \
     const dim3 grid((unsigned)(rows * a.bpr)), block(BT);                                  \
     if (cplx) {                                                                            \
-      if (kind == K_PSD) hipLaunchKernelGGL((k_mean<NV, E, K_PSD, true>), grid, block, 0, s, a); \
-      else if (kind == K_CSD) hipLaunchKernelGGL((k_mean<NV, E, K_CSD, true>), grid, block, 0, s, a); \
-      else hipLaunchKernelGGL((k_mean<NV, E, K_COH, true>), grid, block, 0, s, a);         \
+      MSIG_KIND_SWITCH(MSIG_LAUNCH_MEAN, NV, K_PSD, K_CSD, K_COH, true, false)             \
+    } else if (half) {                                                                     \
+      MSIG_KIND_SWITCH(MSIG_LAUNCH_MEAN, NV, K_PSD, K_CSD, K_COH, false, true)             \
     } else {                                                                               \
-      if (kind == K_PSD) hipLaunchKernelGGL((k_mean<NV, E, K_PSD, false>), grid, block, 0, s, a); \
-      else if (kind == K_CSD) hipLaunchKernelGGL((k_mean<NV, E, K_CSD, false>), grid, block, 0, s, a); \
-      else hipLaunchKernelGGL((k_mean<NV, E, K_COH, false>), grid, block, 0, s, a);        \
+      MSIG_KIND_SWITCH(MSIG_LAUNCH_MEAN, NV, K_PSD, K_CSD, K_COH, false, false)            \
     }                                                                                      \
     return hipGetLastError();                                                              \
   }                                                                                        \
-  hipError_t launch_frames_##NV(int ok, bool cplx, const KArgs& a, int64_t rows,            \
-                               hipStream_t s) {                                            \
-    constexpr int E = NV < 16 ? NV : 16;                                                   \
+  hipError_t launch_frames_##NV(int kind, bool cplx, bool half, const KArgs& a,            \
+                                int64_t rows, hipStream_t s) {                             \
     const dim3 grid((unsigned)(rows * a.bpr)), block(BT);                                  \
     if (cplx) {                                                                            \
-      if (ok == F_COMPLEX) hipLaunchKernelGGL((k_frames<NV, E, F_COMPLEX, true>), grid, block, 0, s, a); \
-      else if (ok == F_POWER) hipLaunchKernelGGL((k_frames<NV, E, F_POWER, true>), grid, block, 0, s, a); \
-      else if (ok == F_CSD) hipLaunchKernelGGL((k_frames<NV, E, F_CSD, true>), grid, block, 0, s, a); \
-      else hipLaunchKernelGGL((k_frames<NV, E, F_COH, true>), grid, block, 0, s, a);       \
+      if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES(NV, F_COMPLEX, true, false);               \
+      else MSIG_KIND_SWITCH(MSIG_LAUNCH_FRAMES, NV, F_POWER, F_CSD, F_COH, true, false)    \
+    } else if (half) {                                                                     \
+      if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES(NV, F_COMPLEX, false, true);               \
+      else MSIG_KIND_SWITCH(MSIG_LAUNCH_FRAMES, NV, F_POWER, F_CSD, F_COH, false, true)    \
     } else {                                                                               \
-      if (ok == F_COMPLEX) hipLaunchKernelGGL((k_frames<NV, E, F_COMPLEX, false>), grid, block, 0, s, a); \
-      else if (ok == F_POWER) hipLaunchKernelGGL((k_frames<NV, E, F_POWER, false>), grid, block, 0, s, a); \
-      else if (ok == F_CSD) hipLaunchKernelGGL((k_frames<NV, E, F_CSD, false>), grid, block, 0, s, a); \
-      else hipLaunchKernelGGL((k_frames<NV, E, F_COH, false>), grid, block, 0, s, a);      \
+      if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES(NV, F_COMPLEX, false, false);              \
+      else MSIG_KIND_SWITCH(MSIG_LAUNCH_FRAMES, NV, F_POWER, F_CSD, F_COH, false, false)   \
     }                                                                                      \
     return hipGetLastError();                                                              \
   }                                                                                        \

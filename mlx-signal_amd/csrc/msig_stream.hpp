@@ -1,0 +1,130 @@
+// Kernel arguments and the prefetching unit stream, host-compilable so the
+// CPU emulator (tests/native/fft_emu.cpp) drives the same code the kernels do.
+#pragma once
+
+#include <stdint.h>
+
+#include "msig_fft.hpp"
+
+namespace msig {
+
+struct KArgs {
+  const void* x;
+  const void* y;
+  const float* win;
+  const cf* twt;     // exp(-2 pi i e / N), e < N
+  float* ws;         // mean kinds: [rows][bpr][ncomp][N] block partials
+  void* out0;        // frames kinds
+  void* out1;
+  void* out2;
+  int64_t ld_x, ld_y;
+  int64_t nseg;
+  int32_t nperseg, hop;
+  int32_t units, run, bpr;
+  int32_t detrend;
+  int32_t nbins;     // bins written per output row / frame
+  int32_t apply_factor;
+  float oscale;      // frames: power scale or sqrt(scale) for complex output
+  // mean kinds: per-row completion tickets (zero before and after a call),
+  // and how the last block of a row turns the partials into outputs
+  int32_t* tickets;
+  int32_t out_kind;  // MSIG_* mean kind
+  int32_t fold;      // real input: fold the two-for-one conjugate mirror
+  double mscale;     // scale / nseg
+};
+
+// Raw samples of consecutive units of one row, fetched one unit ahead so the
+// HBM loads of unit u+1 are in flight while unit u is transformed.
+//
+// General mode: each unit loads its own segment(s) (overlap re-reads hit
+// L1/L2).  HALF mode (real input, nperseg == N, hop == N/2): segments s and
+// s+1 of a pair are half-blocks h_{2u}, h_{2u+1} and h_{2u+1}, h_{2u+2}, and
+// h_{2u+2} is also the first half of the next pair, so each sample is loaded
+// into registers exactly once per group.  Units past the row's end, and the
+// missing partner of an odd segment count, load zeros.
+template <int N, int E, bool CPLX, bool HALF>
+struct UnitStream {
+  static_assert(!(CPLX && HALF), "half-hop reuse is for real input");
+  static constexpr int TPF = N / E;
+  static constexpr int H = E / 2;
+  const char* row;
+  int t;
+  bool pair_ok;  // HALF: segment s0+1 of the fetched unit exists
+  cf raw[HALF ? 1 : E];
+  float h0[HALF ? H : 1], h1[HALF ? H : 1], h2[HALF ? H : 1];
+
+  MSIG_HD void half_load(float* dst, int64_t h, bool valid) {
+    const float* p = reinterpret_cast<const float*>(row) + h * (N / 2);
+    if (valid) {
+#pragma unroll
+      for (int e = 0; e < H; ++e) dst[e] = p[t + TPF * e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < H; ++e) dst[e] = 0.f;
+    }
+  }
+
+  // first == true: nothing carried from a previous fetch
+  MSIG_HD void fetch(int unit, bool first, const KArgs& a) {
+    const bool active = unit < a.units;
+    if constexpr (HALF) {
+      const int64_t s0 = 2 * (int64_t)unit;
+      const bool v1 = active && (s0 + 1 < a.nseg);
+      pair_ok = v1;
+      if (first) {
+        half_load(h0, s0, active);
+      } else {
+#pragma unroll
+        for (int e = 0; e < H; ++e) h0[e] = active ? h2[e] : 0.f;
+      }
+      half_load(h1, s0 + 1, active);
+      half_load(h2, s0 + 2, v1);
+    } else if constexpr (!CPLX) {
+      const float* xrow = reinterpret_cast<const float*>(row);
+      const int64_t s0 = 2 * (int64_t)unit;
+      const bool v1 = active && (s0 + 1 < a.nseg);
+      const float* p0 = xrow + s0 * a.hop;
+      const float* p1 = p0 + a.hop;
+      if (a.nperseg == N && v1) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          raw[e].re = p0[t + TPF * e];
+          raw[e].im = p1[t + TPF * e];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int n = t + TPF * e;
+          const bool in = n < a.nperseg;
+          raw[e].re = (active && in) ? p0[n] : 0.f;
+          raw[e].im = (v1 && in) ? p1[n] : 0.f;
+        }
+      }
+    } else {
+      const cf* p0 = reinterpret_cast<const cf*>(row) + (int64_t)unit * a.hop;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int n = t + TPF * e;
+        raw[e] = (active && n < a.nperseg) ? p0[n] : cf{0.f, 0.f};
+      }
+    }
+  }
+
+  // the fetched unit as complex points z[e] <-> sample t + TPF*e
+  MSIG_HD void take(cf* z) const {
+    if constexpr (HALF) {
+      // a missing partner segment (odd segment count) is all zeros, including
+      // the half it would share with segment s0
+#pragma unroll
+      for (int e = 0; e < H; ++e) {
+        z[e] = cf{h0[e], pair_ok ? h1[e] : 0.f};
+        z[e + H] = cf{h1[e], h2[e]};
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) z[e] = raw[e];
+    }
+  }
+};
+
+}  // namespace msig

@@ -60,6 +60,22 @@ def eligible(x, nperseg: int, nfft: int, sides: str, detrend) -> bool:
                                   int(_is_const(detrend)), int(cplx)))
 
 
+# One workspace per (device, stream).  Its leading ticket region must be zero
+# before a mean call and the kernel leaves it zero, so the buffer is zeroed
+# once when created and reused; calls on one stream are ordered, and calls on
+# different streams use different buffers.
+_WORKSPACES: dict = {}
+
+
+def _workspace(dev: torch.device, nbytes: int) -> torch.Tensor:
+    key = (dev.index, torch.cuda.current_stream(dev).cuda_stream)
+    buf = _WORKSPACES.get(key)
+    if buf is None or buf.numel() < nbytes:
+        buf = torch.zeros(max(nbytes, 1 << 20), dtype=torch.uint8, device=dev)
+        _WORKSPACES[key] = buf
+    return buf
+
+
 def _rows(t: torch.Tensor) -> torch.Tensor:
     if t.stride(-1) != 1 or (t.dim() == 2 and t.stride(0) < t.shape[1]):
         t = t.contiguous()
@@ -109,7 +125,7 @@ def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, oneside
     dev = x2.device
     ws_bytes = ctypes.c_size_t(0)
     _hip.check(lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(ws_bytes)), kind)
-    ws = torch.empty(max(1, ws_bytes.value), dtype=torch.uint8, device=dev)
+    ws = _workspace(dev, ws_bytes.value)
     f32 = dict(dtype=torch.float32, device=dev)
     c64 = dict(dtype=torch.complex64, device=dev)
     if code == _hip.PSD_MEAN or code == _hip.COHERENCE:
@@ -122,7 +138,7 @@ def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, oneside
     o = list(outs) + [None] * (3 - len(outs))
     rc = lib.msig_spectral_mean(ctypes.byref(job), _hip.ptr(x2), _hip.ptr(y2), _hip.ptr(win),
                                 _hip.ptr(o[0]), _hip.ptr(o[1]), _hip.ptr(o[2]), _hip.ptr(ws),
-                                ws_bytes.value, _hip.stream_of(x2))
+                                ws.numel(), _hip.stream_of(x2))
     _hip.check(rc, f"spectral_mean({kind})")
     launches["mean_" + kind] += 1
     return outs
@@ -180,7 +196,8 @@ def _ws(kind, rows, n, nperseg, hop, dev):
     lib = _hip.load()
     _hip.check(lib.msig_mean_workspace_bytes(kind, rows, n, nperseg, hop, ctypes.byref(sz)),
                "workspace")
-    return torch.empty(max(1, sz.value), dtype=torch.uint8, device=dev), sz.value
+    ws = _workspace(dev, sz.value)
+    return ws, ws.numel()
 
 
 def rfft_frames(x, win, nperseg: int, hop: int, detrend, power: bool):

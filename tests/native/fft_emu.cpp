@@ -8,7 +8,12 @@
 //      |X_s|^2 + |X_{s+1}|^2 == (|Z[k]|^2 + |Z[N-k]|^2) / 2, and
 //      conj(X_s)Y_s + conj(X_{s+1})Y_{s+1} == (Q[k] + conj Q[N-k]) / 2
 //      with Q = conj(Zx) Zy.
-// Prints one line per size and exits non-zero on any failure.
+//   4. the kernels' UnitStream (general and hop == N/2 register-reuse modes)
+//      fed through the emulated FFT, detrend, window, accumulate and mirror
+//      fold, against an FP64 Welch mean on the same float32 row, for odd and
+//      even segment counts and nperseg < nfft.  The row is followed by NaN
+//      sentinels so any read past its end poisons the result.
+// Prints one line per case and exits non-zero on any failure.
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -16,6 +21,7 @@
 #include <vector>
 
 #include "msig_fft.hpp"
+#include "msig_stream.hpp"
 
 using msig::cf;
 using cd = std::complex<double>;
@@ -157,6 +163,89 @@ static int check_size(std::mt19937_64& rng) {
   return ok ? 0 : 1;
 }
 
+
+// Welch mean of one row exactly as k_mean does it for a group of TPF threads.
+template <int N, bool HALF>
+static int check_stream(std::mt19937_64& rng, int nperseg, int hop, int n, int detrend) {
+  constexpr int E = N < 16 ? N : 16;
+  constexpr int TPF = N / E;
+  std::normal_distribution<double> nd;
+  std::vector<float> x(n + 2 * N, NAN);
+  for (int i = 0; i < n; ++i) x[i] = (float)(nd(rng) + 0.5 * std::sin(0.3 * i) + 2.0);
+  std::vector<cf> table(N);
+  for (int i = 0; i < N; ++i) {
+    const double ang = -2.0 * M_PI * i / N;
+    table[i] = cf{(float)std::cos(ang), (float)std::sin(ang)};
+  }
+  std::vector<double> wd(nperseg);
+  for (int i = 0; i < nperseg; ++i) wd[i] = 0.5 - 0.5 * std::cos(2.0 * M_PI * i / nperseg);
+  msig::KArgs a{};
+  a.nseg = (n - nperseg) / hop + 1;
+  a.nperseg = nperseg;
+  a.hop = hop;
+  a.units = (int32_t)((a.nseg + 1) / 2);
+  a.detrend = detrend;
+  std::vector<msig::UnitStream<N, E, false, HALF>> st(TPF);
+  for (int t = 0; t < TPF; ++t) {
+    st[t].row = reinterpret_cast<const char*>(x.data());
+    st[t].t = t;
+    st[t].fetch(0, true, a);
+  }
+  std::vector<double> S(N, 0.0);
+  for (int u = 0; u <= a.units; ++u) {  // one unit past the end: must load zeros
+    std::vector<cf> z(N), Z;
+    for (int t = 0; t < TPF; ++t) {
+      cf zt[E];
+      st[t].take(zt);
+      for (int e = 0; e < E; ++e) z[t + TPF * e] = zt[e];
+      if (u < a.units) st[t].fetch(u + 1, false, a);
+    }
+    if (u == a.units) {
+      for (int i = 0; i < N; ++i)
+        if (z[i].re != 0.f || z[i].im != 0.f) {
+          std::printf("stream N=%d: unit past the end is not zero\n", N);
+          return 1;
+        }
+      break;
+    }
+    float sr = 0.f, si = 0.f;
+    for (int i = 0; i < N; ++i) {
+      sr += z[i].re;
+      si += z[i].im;
+    }
+    const float mr = detrend ? sr / nperseg : 0.f, mi = detrend ? si / nperseg : 0.f;
+    for (int i = 0; i < N; ++i) {
+      const float w = i < nperseg ? (float)wd[i] : 0.f;
+      z[i] = cf{(z[i].re - mr) * w, (z[i].im - mi) * w};
+    }
+    if (!emu_fft<N, E>(table, z, Z)) return 1;
+    for (int k = 0; k < N; ++k) S[k] += (double)Z[k].re * Z[k].re + (double)Z[k].im * Z[k].im;
+  }
+  // FP64 reference: per-segment detrend, window, zero-pad to N, DFT
+  std::vector<double> P(N, 0.0);
+  for (int s = 0; s < a.nseg; ++s) {
+    std::vector<cd> seg(N, 0.0);
+    double m = 0;
+    for (int i = 0; i < nperseg; ++i) m += x[(size_t)s * hop + i];
+    m = detrend ? m / nperseg : 0.0;
+    for (int i = 0; i < nperseg; ++i) seg[i] = (x[(size_t)s * hop + i] - m) * wd[i];
+    auto X = dft(seg);
+    for (int k = 0; k < N; ++k) P[k] += std::norm(X[k]);
+  }
+  double peak = 0, err = 0;
+  for (int k = 0; k < N; ++k) {
+    const double got = 0.5 * (S[k] + S[(N - k) & (N - 1)]);
+    peak = std::max(peak, P[k]);
+    err = std::max(err, std::abs(got - P[k]));
+  }
+  const double rel = std::isfinite(err) ? err / peak : 1.0;
+  const bool ok = rel < 1e-5;
+  std::printf("stream N=%5d %s nperseg=%d hop=%d nseg=%lld detrend=%d rel=%.3e %s\n", N,
+              HALF ? "half" : "gen ", nperseg, hop, (long long)a.nseg, detrend, rel,
+              ok ? "ok" : "FAIL");
+  return ok ? 0 : 1;
+}
+
 int main() {
   std::mt19937_64 rng(1234);
   int bad = 0;
@@ -169,5 +258,15 @@ int main() {
   bad += check_size<1024>(rng);
   bad += check_size<2048>(rng);
   bad += check_size<4096>(rng);
+  // hop == N/2 register reuse: odd and even segment counts
+  bad += check_stream<64, true>(rng, 64, 32, 64 + 32 * 14, 1);   // nseg 15
+  bad += check_stream<64, true>(rng, 64, 32, 64 + 32 * 15, 0);   // nseg 16
+  bad += check_stream<256, true>(rng, 256, 128, 256 + 128 * 8 + 50, 1);
+  bad += check_stream<1024, true>(rng, 1024, 512, 1024 + 512 * 6, 1);
+  // general stream: arbitrary hop, nperseg < nfft
+  bad += check_stream<64, false>(rng, 64, 7, 300, 1);
+  bad += check_stream<64, false>(rng, 50, 20, 333, 1);
+  bad += check_stream<256, false>(rng, 256, 100, 5000, 0);
+  bad += check_stream<256, false>(rng, 200, 200, 4000, 1);
   return bad ? 1 : 0;
 }
