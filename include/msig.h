@@ -22,11 +22,10 @@
  *   per-frame outputs    [rows][nseg][bins]
  *   complex outputs      interleaved (re, im) float32 pairs (= complex64)
  *
- * Mean-kind workspace: the first ceil(rows * 4 / 256) * 256 bytes are per-row
- * completion tickets.  They must be zero before the first call on a
- * workspace; every call leaves them zero again, so one workspace can be
- * reused by successive calls on one stream without clearing.  Calls that
- * overlap in time must use different workspaces.
+ * Mean kinds run two kernels on the stream: the fused segment kernel writes
+ * per-block partial sums into the caller's workspace and a finalize kernel
+ * reduces them.  The workspace needs no initialisation; calls that overlap
+ * in time must use different workspaces.
  */
 #ifndef MSIG_H
 #define MSIG_H

@@ -72,6 +72,93 @@ static int twiddles(int N, const cf** out) {
   return 0;
 }
 
+// ---------------------------------------------------------------- finalize
+struct FinArgs {
+  const float* ws;
+  void* out0;
+  void* out1;
+  void* out2;
+  int64_t rows;
+  int32_t bpr, N, nbins, nc, kind, fold, apply_factor;
+  double scale;  // scale / nseg
+};
+
+constexpr int FIN_BINS = 64;   // output bins per block
+constexpr int FIN_LANES = 4;   // threads sharing one bin's block partials
+
+// Sum the block partials of each row in FP64, fold the conjugate mirror of
+// the two-for-one packing, apply scale, 1/nseg and the onesided factor.
+// Thread (bin j, lane q) sums partials q, q+4, q+8, ... with all its loads in
+// flight together; the four lane sums are then added in lane order, so the
+// reduction order is fixed by the job's geometry.  Counterpart of
+// _reduce_groups (_stft_metal.py:443-455) and the deferred onesided factor of
+// csd (spectral.py:482-488).
+__global__ __launch_bounds__(FIN_BINS * FIN_LANES) void k_finalize(const FinArgs f) {
+  __shared__ double part[FIN_LANES][4][FIN_BINS][2];
+  const int chunks = (f.nbins + FIN_BINS - 1) / FIN_BINS;
+  const int64_t row = blockIdx.x / chunks;
+  const int j = threadIdx.x % FIN_BINS, q = threadIdx.x / FIN_BINS;
+  const int k = (int)(blockIdx.x % chunks) * FIN_BINS + j;
+  const int N = f.N;
+  const int kk = k < f.nbins ? k : 0;
+  const int mk = (N - kk) & (N - 1);
+  const float* base = f.ws + (size_t)row * f.bpr * f.nc * N;
+  const size_t bstride = (size_t)f.nc * N;
+  for (int c = 0; c < f.nc; ++c) {
+    double s = 0.0, m = 0.0;
+    int b = q;
+    for (; b + 3 * FIN_LANES < f.bpr; b += 4 * FIN_LANES) {
+      float v[4], w[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float* pb = base + (size_t)(b + i * FIN_LANES) * bstride + (size_t)c * N;
+        v[i] = pb[kk];
+        w[i] = f.fold ? pb[mk] : 0.f;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        s += (double)v[i];
+        m += (double)w[i];
+      }
+    }
+    for (; b < f.bpr; b += FIN_LANES) {
+      const float* pb = base + (size_t)b * bstride + (size_t)c * N;
+      s += (double)pb[kk];
+      if (f.fold) m += (double)pb[mk];
+    }
+    part[q][c][j][0] = s;
+    part[q][c][j][1] = m;
+  }
+  __syncthreads();
+  if (q != 0 || k >= f.nbins) return;
+  double p[4] = {0, 0, 0, 0};
+  for (int c = 0; c < f.nc; ++c) {
+    double s = 0.0, m = 0.0;
+    for (int l = 0; l < FIN_LANES; ++l) {
+      s += part[l][c][j][0];
+      m += part[l][c][j][1];
+    }
+    // powers and real parts fold symmetric, imaginary parts antisymmetric
+    const bool imag = (f.nc == 2 && c == 1) || (f.nc == 4 && c == 3);
+    p[c] = f.fold ? 0.5 * (imag ? s - m : s + m) : s;
+  }
+  double sc = f.scale;
+  if (f.apply_factor && !(k == 0 || 2 * k == N)) sc *= 2.0;
+  const size_t o = (size_t)row * f.nbins + k;
+  if (f.kind == MSIG_PSD_MEAN) {
+    static_cast<float*>(f.out0)[o] = (float)(p[0] * sc);
+  } else if (f.kind == MSIG_CSD_MEAN) {
+    static_cast<cf*>(f.out0)[o] = cf{(float)(p[0] * sc), (float)(p[1] * sc)};
+  } else if (f.kind == MSIG_COH_TRIO_MEAN) {
+    static_cast<float*>(f.out0)[o] = (float)(p[0] * sc);
+    static_cast<float*>(f.out1)[o] = (float)(p[1] * sc);
+    static_cast<cf*>(f.out2)[o] = cf{(float)(p[2] * sc), (float)(p[3] * sc)};
+  } else {  // MSIG_COHERENCE: scales cancel in the ratio
+    const double num = p[2] * p[2] + p[3] * p[3];
+    static_cast<float*>(f.out0)[o] = (float)(num / p[0] / p[1]);
+  }
+}
+
 // ---------------------------------------------------------------- dispatch
 static bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
@@ -153,14 +240,11 @@ static int nbins_of(const msig_job* j) {
   return (!cplx && (j->flags & MSIG_F_ONESIDED)) ? j->nfft / 2 + 1 : j->nfft;
 }
 
-// Mean-kind workspace: per-row tickets (int32, padded to 256 bytes) followed
-// by the block partials [rows][bpr][ncomp][nfft] float32.
-static size_t ticket_bytes(int64_t rows) { return ((size_t)rows * sizeof(int32_t) + 255) / 256 * 256; }
-
+// Mean-kind workspace: the block partials [rows][bpr][ncomp][nfft] float32.
 static size_t mean_ws_bytes(const msig_job* j) {
   const bool cplx = j->flags & MSIG_F_COMPLEX_INPUT;
   const Geo g = geometry(j->rows, j->nseg, j->nfft, cplx, MEAN_MAX_RUN);
-  return ticket_bytes(j->rows) + (size_t)j->rows * g.bpr * ncomp(j->kind) * j->nfft * sizeof(float);
+  return (size_t)j->rows * g.bpr * ncomp(j->kind) * j->nfft * sizeof(float);
 }
 
 static KArgs base_args(const msig_job* j, const void* x, const void* y, const float* win,
@@ -250,20 +334,31 @@ int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const 
   rc = twiddles(N, &tw);
   if (rc) return rc;
   KArgs a = base_args(job, x, y, win, tw, g);
-  a.tickets = static_cast<int32_t*>(workspace);
-  a.ws = reinterpret_cast<float*>(static_cast<char*>(workspace) + ticket_bytes(job->rows));
-  a.out0 = out0;
-  a.out1 = out1;
-  a.out2 = out2;
-  a.nbins = nbins_of(job);
-  a.out_kind = job->kind;
-  a.fold = cplx ? 0 : 1;
-  a.apply_factor =
-      (!cplx && (job->flags & MSIG_F_APPLY_FACTOR) && (job->flags & MSIG_F_ONESIDED)) ? 1 : 0;
-  a.mscale = job->scale / (double)job->nseg;
+  a.ws = static_cast<float*>(workspace);
   const int kk = job->kind == MSIG_PSD_MEAN ? K_PSD : (job->kind == MSIG_CSD_MEAN ? K_CSD : K_COH);
-  hipError_t e = dispatch_mean(N, kk, cplx, a, job->rows, static_cast<hipStream_t>(stream));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = dispatch_mean(N, kk, cplx, a, job->rows, s);
   if (e != hipSuccess) return hip_fail(e, "fused mean kernel launch");
+  FinArgs f;
+  f.ws = a.ws;
+  f.out0 = out0;
+  f.out1 = out1;
+  f.out2 = out2;
+  f.rows = job->rows;
+  f.bpr = g.bpr;
+  f.N = N;
+  f.nbins = nbins_of(job);
+  f.nc = ncomp(job->kind);
+  f.kind = job->kind;
+  f.fold = cplx ? 0 : 1;
+  f.apply_factor =
+      (!cplx && (job->flags & MSIG_F_APPLY_FACTOR) && (job->flags & MSIG_F_ONESIDED)) ? 1 : 0;
+  f.scale = job->scale / (double)job->nseg;
+  const int64_t chunks = (f.nbins + FIN_BINS - 1) / FIN_BINS;
+  const dim3 grid((unsigned)(job->rows * chunks)), block(FIN_BINS * FIN_LANES);
+  hipLaunchKernelGGL(k_finalize, grid, block, 0, s, f);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "finalize kernel launch");
   return 0;
 }
 

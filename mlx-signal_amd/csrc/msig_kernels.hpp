@@ -150,73 +150,6 @@ __device__ __forceinline__ void prep_unit(cf* z, const float* w, int detrend, in
   }
 }
 
-// FP64 sum of a row's block partials in block order, conjugate-mirror fold
-// of the two-for-one packing, scale / nseg and the onesided factor.
-// Counterpart of _reduce_groups (_stft_metal.py:443-455) and the deferred
-// onesided factor of csd (spectral.py:482-488).
-template <int NC>
-__device__ __forceinline__ void finalize_row(const KArgs& a, int N, int64_t row, int tid) {
-  const float* base = a.ws + (size_t)row * a.bpr * NC * N;
-  const size_t bstride = (size_t)NC * N;
-  for (int k = tid; k < a.nbins; k += BT) {
-    const int mk = (N - k) & (N - 1);
-    double s[NC], m[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      s[c] = 0.0;
-      m[c] = 0.0;
-    }
-    int b = 0;
-    for (; b + 4 <= a.bpr; b += 4) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        float v[4], w[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float* pb = base + (b + i) * bstride + (size_t)c * N;
-          v[i] = __builtin_nontemporal_load(pb + k);
-          w[i] = a.fold ? __builtin_nontemporal_load(pb + mk) : 0.f;
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          s[c] += (double)v[i];
-          m[c] += (double)w[i];
-        }
-      }
-    }
-    for (; b < a.bpr; ++b) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const float* pb = base + b * bstride + (size_t)c * N;
-        s[c] += (double)__builtin_nontemporal_load(pb + k);
-        if (a.fold) m[c] += (double)__builtin_nontemporal_load(pb + mk);
-      }
-    }
-    double p[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      // powers and real parts fold symmetric, imaginary parts antisymmetric
-      const bool imag = (NC == 2 && c == 1) || (NC == 4 && c == 3);
-      p[c] = a.fold ? 0.5 * (imag ? s[c] - m[c] : s[c] + m[c]) : s[c];
-    }
-    double sc = a.mscale;
-    if (a.apply_factor && !(k == 0 || 2 * k == N)) sc *= 2.0;
-    const size_t o = (size_t)row * a.nbins + k;
-    if (a.out_kind == 0) {  // MSIG_PSD_MEAN
-      static_cast<float*>(a.out0)[o] = (float)(p[0] * sc);
-    } else if (a.out_kind == 1) {  // MSIG_CSD_MEAN
-      static_cast<cf*>(a.out0)[o] = cf{(float)(p[0] * sc), (float)(p[1] * sc)};
-    } else if (a.out_kind == 2) {  // MSIG_COH_TRIO_MEAN
-      static_cast<float*>(a.out0)[o] = (float)(p[0] * sc);
-      static_cast<float*>(a.out1)[o] = (float)(p[1] * sc);
-      static_cast<cf*>(a.out2)[o] = cf{(float)(p[2] * sc), (float)(p[3] * sc)};
-    } else {  // MSIG_COHERENCE: scale and factor cancel in the ratio
-      const double num = p[2] * p[2] + p[3] * p[3];
-      static_cast<float*>(a.out0)[o] = (float)(num / p[0] / p[1]);
-    }
-  }
-}
-
 // Grouped mean over segments: per-bin sums of |Zx|^2 (PSD), conj(Zx) Zy
 // (CSD) or both spectra and the cross term (COH), written per block.
 template <int N, int E, int KIND, bool CPLX, bool HALF>
@@ -276,7 +209,7 @@ __device__ __forceinline__ void mean_body(const KArgs& a, cf* lds_all, float* re
       fft_passes<N, E, TH::TS>(t, zx, tw, lds);
       if (active) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) acc[0][e] += zx[e].re * zx[e].re + zx[e].im * zx[e].im;
+        for (int e = 0; e < E; ++e) acc[0][e] = fmaf(zx[e].im, zx[e].im, fmaf(zx[e].re, zx[e].re, acc[0][e]));
       }
     } else {
       cf zy[E];
@@ -293,15 +226,19 @@ __device__ __forceinline__ void mean_body(const KArgs& a, cf* lds_all, float* re
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const cf q = cmulc(zx[e], zy[e]);
+          // conj(zx) zy accumulated with four FMAs per bin
+          float qr = fmaf(zx[e].re, zy[e].re, acc[K_CSD == KIND ? 0 : 2][e]);
+          float qi = fmaf(zx[e].re, zy[e].im, acc[K_CSD == KIND ? 1 : 3][e]);
+          qr = fmaf(zx[e].im, zy[e].im, qr);
+          qi = fmaf(-zx[e].im, zy[e].re, qi);
           if constexpr (KIND == K_CSD) {
-            acc[0][e] += q.re;
-            acc[1][e] += q.im;
+            acc[0][e] = qr;
+            acc[1][e] = qi;
           } else {
-            acc[0][e] += zx[e].re * zx[e].re + zx[e].im * zx[e].im;
-            acc[1][e] += zy[e].re * zy[e].re + zy[e].im * zy[e].im;
-            acc[2][e] += q.re;
-            acc[3][e] += q.im;
+            acc[0][e] = fmaf(zx[e].im, zx[e].im, fmaf(zx[e].re, zx[e].re, acc[0][e]));
+            acc[1][e] = fmaf(zy[e].im, zy[e].im, fmaf(zy[e].re, zy[e].re, acc[1][e]));
+            acc[2][e] = qr;
+            acc[3][e] = qi;
           }
         }
       }
@@ -326,19 +263,6 @@ __device__ __forceinline__ void mean_body(const KArgs& a, cf* lds_all, float* re
     }
   }
 
-  // The last block of a row to finish reduces the row's partials
-  // (threadFence-reduction pattern): publish, take a ticket, and if it is
-  // the row's final ticket, read every block's partial back from L2.
-  __threadfence();
-  __syncthreads();
-  __shared__ int is_last;
-  if (tid == 0) is_last = (atomicAdd(a.tickets + row, 1) == a.bpr - 1);
-  __syncthreads();
-  if (is_last) {
-    __threadfence();
-    finalize_row<NC>(a, N, row, tid);
-    if (tid == 0) a.tickets[row] = 0;  // leave the ticket zeroed for the next call
-  }
 }
 
 // Waves per SIMD each kernel is compiled for.  Single-signal kinds fit three

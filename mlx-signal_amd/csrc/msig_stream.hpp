@@ -25,12 +25,6 @@ struct KArgs {
   int32_t nbins;     // bins written per output row / frame
   int32_t apply_factor;
   float oscale;      // frames: power scale or sqrt(scale) for complex output
-  // mean kinds: per-row completion tickets (zero before and after a call),
-  // and how the last block of a row turns the partials into outputs
-  int32_t* tickets;
-  int32_t out_kind;  // MSIG_* mean kind
-  int32_t fold;      // real input: fold the two-for-one conjugate mirror
-  double mscale;     // scale / nseg
 };
 
 // Raw samples of consecutive units of one row, fetched one unit ahead so the

@@ -60,10 +60,9 @@ def eligible(x, nperseg: int, nfft: int, sides: str, detrend) -> bool:
                                   int(_is_const(detrend)), int(cplx)))
 
 
-# One workspace per (device, stream).  Its leading ticket region must be zero
-# before a mean call and the kernel leaves it zero, so the buffer is zeroed
-# once when created and reused; calls on one stream are ordered, and calls on
-# different streams use different buffers.
+# One partial-sum workspace per (device, stream), grown on demand and reused:
+# calls on one stream are ordered, calls on different streams use different
+# buffers.
 _WORKSPACES: dict = {}
 
 
@@ -71,7 +70,7 @@ def _workspace(dev: torch.device, nbytes: int) -> torch.Tensor:
     key = (dev.index, torch.cuda.current_stream(dev).cuda_stream)
     buf = _WORKSPACES.get(key)
     if buf is None or buf.numel() < nbytes:
-        buf = torch.zeros(max(nbytes, 1 << 20), dtype=torch.uint8, device=dev)
+        buf = torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, device=dev)
         _WORKSPACES[key] = buf
     return buf
 
