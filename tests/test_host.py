@@ -1,0 +1,199 @@
+"""Host-side contract of the drop-in API (CPU only): configuration, dispatch
+routing, dtype policy, argument validation and empty-input shapes, following
+the reference suite (tests/test_dispatch_dtypes.py, tests/test_edge_cases.py)."""
+
+import warnings
+
+import numpy as np
+import pytest
+import scipy.signal as sps
+import torch
+
+import mlx_signal_amd as msig
+from _utils import assert_close, make_sig
+from mlx_signal_amd import _array, _device, spectral
+
+
+def test_set_config_is_atomic():
+    before = msig.get_config()
+    with pytest.raises(ValueError):
+        msig.set_config(dispatch="scipy", float64="never")
+    assert msig.get_config() == before
+    with pytest.raises(ValueError):
+        msig.set_config(gpu_min_size=-1)
+    assert msig.get_config() == before
+    with pytest.raises(TypeError):
+        msig.set_config(nonsense=1)
+    with pytest.raises(ValueError):
+        msig.set_config(dispatch="cuda")
+
+
+def test_config_context_restores():
+    base = msig.get_config().dispatch
+    with msig.config_context(dispatch="scipy"):
+        assert msig.get_config().dispatch == "scipy"
+    assert msig.get_config().dispatch == base
+
+
+def test_mlx_dispatch_alias_accepted():
+    with msig.config_context(dispatch="mlx"):
+        assert msig.get_config().dispatch == "mlx"
+
+
+def test_scipy_route_returns_float32_tensor(rng):
+    x = rng.standard_normal(4096).astype(np.float32)
+    with msig.config_context(dispatch="scipy"):
+        f, p = msig.welch(x, nperseg=256)
+    assert isinstance(p, torch.Tensor) and p.dtype == torch.float32
+    assert_close(p, sps.welch(x, nperseg=256)[1])
+
+
+def test_auto_small_input_routes_to_scipy(rng):
+    x = rng.standard_normal(512).astype(np.float32)
+    with msig.config_context(dispatch="auto", gpu_min_size=1 << 20):
+        f, p = msig.welch(x, nperseg=128)
+    assert isinstance(p, torch.Tensor) and p.dtype == torch.float32
+    assert_close(p, sps.welch(x, nperseg=128)[1])
+
+
+@pytest.mark.skipif(_device.gpu_available(), reason="checks the no-GPU behaviour")
+def test_gpu_dispatch_without_gpu_raises(rng):
+    x = rng.standard_normal(4096).astype(np.float32)
+    with msig.config_context(dispatch="gpu"):
+        with pytest.raises(RuntimeError, match="GPU"):
+            msig.welch(x, nperseg=256)
+
+
+def test_callable_detrend_falls_back_with_warning(rng):
+    x = rng.standard_normal(1 << 16).astype(np.float32)
+    with msig.config_context(dispatch="auto", gpu_min_size=0):
+        if _device.gpu_available():
+            with pytest.warns(msig.FallbackWarning):
+                msig.welch(x, nperseg=256, detrend=lambda d: d - d.mean(-1, keepdims=True))
+        else:
+            msig.welch(x, nperseg=256, detrend=lambda d: d)
+
+
+def test_callable_detrend_under_gpu_dispatch_raises(rng):
+    x = rng.standard_normal(1 << 16).astype(np.float32)
+    with msig.config_context(dispatch="gpu"):
+        with pytest.raises(NotImplementedError):
+            msig.welch(x, nperseg=256, detrend=lambda d: d)
+
+
+def test_float64_strict_raises_on_scipy_path(rng):
+    x = rng.standard_normal(100)
+    with msig.config_context(dispatch="scipy", float64="strict"):
+        with pytest.raises(TypeError, match="strict"):
+            msig.welch(x, nperseg=32)
+
+
+def test_dtype_policy_to_device():
+    dev = torch.device("cpu")
+    with msig.config_context(float64="downcast", warn_on_downcast=True):
+        with pytest.warns(msig.DowncastWarning):
+            t = _array.to_device(np.zeros(8), dev)
+        assert t.dtype == torch.float32
+        with pytest.warns(msig.DowncastWarning):
+            t = _array.to_device(np.zeros(8, np.complex128), dev)
+        assert t.dtype == torch.complex64
+    with warnings.catch_warnings():
+        warnings.simplefilter("error", msig.DowncastWarning)
+        assert _array.to_device([1.0, 2.0], dev).dtype == torch.float32
+        assert _array.to_device(np.arange(5, dtype=np.int32), dev).dtype == torch.float32
+    with msig.config_context(float64="strict"):
+        with pytest.raises(TypeError, match="strict"):
+            _array.to_device(np.zeros(4), dev)
+        with pytest.raises(TypeError):
+            _array.check_strict(torch.zeros(3, dtype=torch.float64))
+
+
+@pytest.mark.parametrize("func", ["welch", "csd"])
+def test_empty_input_shapes(func, rng):
+    x = np.array([], dtype=np.float32)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        if func == "welch":
+            f_ref, p_ref = sps.welch(x)
+            f, p = msig.welch(x)
+        else:
+            y = rng.standard_normal(0).astype(np.float32)
+            f_ref, p_ref = sps.csd(x, y)
+            f, p = msig.csd(x, y)
+    assert tuple(f.shape) == f_ref.shape
+    assert tuple(p.shape) == p_ref.shape
+
+
+def test_argument_errors():
+    x = np.zeros(4096, np.float32)
+    with pytest.raises(ValueError, match="average"):
+        msig.csd(x, x, average="mode")
+    with pytest.raises(ValueError, match="scaling"):
+        msig.stft(x, scaling="energy")
+    with pytest.raises(ValueError, match="mode"):
+        msig.spectrogram(x, mode="power")
+    with pytest.raises(ValueError, match="mode"):
+        spectral._spectral_helper(x, x, mode="bad")
+    with pytest.raises(ValueError, match="boundary"):
+        spectral._spectral_helper(x, x, boundary="wrap")
+    with pytest.raises(ValueError, match="equal"):
+        spectral._spectral_helper(x, np.zeros(4096, np.float32), mode="stft")
+
+
+def test_triage_segments_matches_reference_rules():
+    w, n = spectral._triage_segments("hann", None, 1000)
+    assert n == 256 and w.shape == (256,)
+    with pytest.warns(UserWarning, match="nperseg"):
+        w, n = spectral._triage_segments("hann", 512, 100)
+    assert n == 100
+    with pytest.raises(ValueError, match="1-D"):
+        spectral._triage_segments(np.ones((2, 2)), None, 100)
+    with pytest.raises(ValueError, match="longer"):
+        spectral._triage_segments(np.ones(200), None, 100)
+    with pytest.raises(ValueError, match="different"):
+        spectral._triage_segments(np.ones(64), 32, 100)
+    np.testing.assert_array_equal(spectral._onesided_factor_np(8), [1, 2, 2, 2, 1])
+    np.testing.assert_array_equal(spectral._onesided_factor_np(7), [1, 2, 2, 2])
+    assert abs(spectral._median_bias(3) - 5 / 6) < 1e-15
+
+
+def test_median_last_matches_numpy(rng):
+    a = rng.standard_normal((3, 5, 8)).astype(np.float32)
+    for n in (7, 8):
+        t = torch.from_numpy(a[..., :n])
+        np.testing.assert_allclose(spectral._median_last(t).numpy(), np.median(a[..., :n], axis=-1))
+
+
+def test_unwrap_matches_numpy(rng):
+    p = np.cumsum(rng.uniform(-3, 3, (4, 50)), axis=1).astype(np.float32)
+    for ax in (0, 1):
+        np.testing.assert_allclose(spectral._unwrap(torch.from_numpy(p), dim=ax).numpy(),
+                                   np.unwrap(p, axis=ax), rtol=1e-5, atol=1e-4)
+
+
+def test_edge_extensions_match_scipy(rng):
+    from scipy.signal._arraytools import const_ext, even_ext, odd_ext, zero_ext
+
+    from mlx_signal_amd import _arraytools as at
+
+    x = rng.standard_normal((2, 20)).astype(np.float32)
+    t = torch.from_numpy(x)
+    for ours, ref in ((at.even_ext, even_ext), (at.odd_ext, odd_ext),
+                      (at.const_ext, const_ext), (at.zero_ext, zero_ext)):
+        np.testing.assert_allclose(ours(t, 5).numpy(), ref(x, 5, axis=-1), rtol=1e-6)
+    with pytest.raises(ValueError, match="too big"):
+        at.even_ext(t, 20)
+
+
+def test_composed_detrend_matches_oracle(rng):
+    """The composed path's detrend/window against the oracle's FP64 formulas."""
+    import oracle.spectral_fp64 as osp
+
+    d = make_sig(rng, (3, 7, 64))
+    w = np.hanning(64).astype(np.float32)
+    for kind in (False, "constant", "linear"):
+        got = spectral._detrend_and_window(torch.from_numpy(d), kind, torch.from_numpy(w)).numpy()
+        ref = osp._detrend(d.astype(np.float64), kind) * w
+        np.testing.assert_allclose(got, ref, rtol=1e-4, atol=1e-5)
+    with pytest.raises(ValueError):
+        spectral._detrend_and_window(torch.from_numpy(d), "quadratic", torch.from_numpy(w))
