@@ -149,6 +149,46 @@ int msig_coherence_mean(const float* x, const float* y, int64_t rows, int64_t n,
                         float* pxx, float* pyy, void* pxy, void* workspace, size_t ws_bytes,
                         void* stream);
 
+/* ---- ragged batches (many mean jobs of any shape in one submission) ----
+ * No counterpart in the reference (it launches one kernel per call).  A plan
+ * partitions the jobs' work by FFT size, uploads per-job descriptors and work
+ * queues to the device once, and each run is kernels only (one persistent
+ * launch per size class plus one finalize launch), so a run can be captured
+ * in a hipGraph and replayed.  Every job is computed with the same block
+ * geometry as a standalone msig_spectral_mean call with the same tuning, so
+ * its result is bitwise the same alone and in any batch. */
+
+typedef struct msig_batch_job {
+  msig_job job;     /* a mean kind: MSIG_PSD_MEAN, MSIG_CSD_MEAN, MSIG_COH_TRIO_MEAN, MSIG_COHERENCE */
+  const void* x;
+  const void* y;    /* NULL for MSIG_PSD_MEAN */
+  const float* win; /* nperseg float32 on the device */
+  void* out0;
+  void* out1;
+  void* out2;
+} msig_batch_job;
+
+/* Per FFT size (index log2(nfft)): 0 selects the default.
+ *   max_run:       most segment pairs one thread group sums sequentially (<= 32)
+ *   target_blocks: blocks a job's geometry aims for (default 1024)
+ *   blocks_per_cu: resident blocks per CU of the persistent launch (default 4) */
+typedef struct msig_tuning {
+  int32_t max_run[13];
+  int32_t target_blocks[13];
+  int32_t blocks_per_cu[13];
+} msig_tuning;
+
+typedef struct msig_batch msig_batch; /* opaque plan */
+
+int msig_batch_create(const msig_batch_job* jobs, int32_t njobs, const msig_tuning* tuning,
+                      msig_batch** plan);
+int msig_batch_workspace_bytes(const msig_batch* plan, size_t* bytes);
+/* Launches on `stream`; allocates nothing and copies nothing from the host. */
+int msig_batch_run(const msig_batch* plan, void* workspace, size_t ws_bytes, void* stream);
+/* Number of kernel launches one run makes (size classes + 1). */
+int msig_batch_launches(const msig_batch* plan);
+int msig_batch_destroy(msig_batch* plan);
+
 /* Workspace bytes for the operator-level mean entry points
  * (kind = MSIG_PSD_MEAN, MSIG_CSD_MEAN or MSIG_COH_TRIO_MEAN). */
 int msig_mean_workspace_bytes(int32_t kind, int64_t rows, int64_t n, int32_t nperseg, int32_t hop,

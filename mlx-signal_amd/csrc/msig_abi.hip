@@ -12,6 +12,8 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
+#include <climits>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -80,7 +82,8 @@ struct FinArgs {
   void* out2;
   int64_t rows;
   int32_t bpr, N, nbins, nc, kind, fold, apply_factor;
-  double scale;  // scale / nseg
+  double scale;    // scale / nseg
+  int64_t ws_off;  // batch plans: partials start at ws_base + ws_off
 };
 
 constexpr int FIN_BINS = 64;   // output bins per block
@@ -93,12 +96,13 @@ constexpr int FIN_LANES = 4;   // threads sharing one bin's block partials
 // reduction order is fixed by the job's geometry.  Counterpart of
 // _reduce_groups (_stft_metal.py:443-455) and the deferred onesided factor of
 // csd (spectral.py:482-488).
-__global__ __launch_bounds__(FIN_BINS * FIN_LANES) void k_finalize(const FinArgs f) {
+__device__ __forceinline__ void finalize_write(const FinArgs& f, int64_t row, int k, int j,
+                                               const double (&part)[FIN_LANES][4][FIN_BINS][2]);
+
+__device__ __forceinline__ void finalize_chunk(const FinArgs& f, int64_t row, int chunk) {
   __shared__ double part[FIN_LANES][4][FIN_BINS][2];
-  const int chunks = (f.nbins + FIN_BINS - 1) / FIN_BINS;
-  const int64_t row = blockIdx.x / chunks;
   const int j = threadIdx.x % FIN_BINS, q = threadIdx.x / FIN_BINS;
-  const int k = (int)(blockIdx.x % chunks) * FIN_BINS + j;
+  const int k = chunk * FIN_BINS + j;
   const int N = f.N;
   const int kk = k < f.nbins ? k : 0;
   const int mk = (N - kk) & (N - 1);
@@ -130,7 +134,13 @@ __global__ __launch_bounds__(FIN_BINS * FIN_LANES) void k_finalize(const FinArgs
     part[q][c][j][1] = m;
   }
   __syncthreads();
-  if (q != 0 || k >= f.nbins) return;
+  if (q == 0 && k < f.nbins) finalize_write(f, row, k, j, part);
+  __syncthreads();
+}
+
+__device__ __forceinline__ void finalize_write(const FinArgs& f, int64_t row, int k, int j,
+                                               const double (&part)[FIN_LANES][4][FIN_BINS][2]) {
+  const int N = f.N;
   double p[4] = {0, 0, 0, 0};
   for (int c = 0; c < f.nc; ++c) {
     double s = 0.0, m = 0.0;
@@ -156,6 +166,24 @@ __global__ __launch_bounds__(FIN_BINS * FIN_LANES) void k_finalize(const FinArgs
   } else {  // MSIG_COHERENCE: scales cancel in the ratio
     const double num = p[2] * p[2] + p[3] * p[3];
     static_cast<float*>(f.out0)[o] = (float)(num / p[0] / p[1]);
+  }
+}
+
+__global__ __launch_bounds__(FIN_BINS * FIN_LANES) void k_finalize(const FinArgs f) {
+  const int chunks = (f.nbins + FIN_BINS - 1) / FIN_BINS;
+  finalize_chunk(f, blockIdx.x / chunks, (int)(blockIdx.x % chunks));
+}
+
+// Finalize of a batch plan: a persistent grid over (job, row * chunks + chunk).
+__global__ __launch_bounds__(FIN_BINS * FIN_LANES) void k_finalize_batch(
+    const FinArgs* __restrict__ fins, const WorkItem* __restrict__ queue, int32_t nwork,
+    const float* ws_base) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem item = queue[w];
+    FinArgs f = fins[item.job];
+    f.ws = ws_base + f.ws_off;
+    const int chunks = (f.nbins + FIN_BINS - 1) / FIN_BINS;
+    finalize_chunk(f, item.blk / chunks, item.blk % chunks);
   }
 }
 
@@ -187,6 +215,19 @@ static hipError_t dispatch_frames(int N, int ok, bool cplx, const KArgs& a, int6
 #define C(NV) \
   case NV:    \
     return launch_frames_##NV(ok, cplx, half, a, rows, s);
+    C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+  }
+  return hipErrorInvalidValue;
+}
+
+static hipError_t dispatch_mean_batch(int N, int kk, bool cplx, bool half, const KArgs* jobs,
+                                      const WorkItem* queue, int32_t nwork, float* ws, int grid,
+                                      hipStream_t s) {
+  switch (N) {
+#define C(NV) \
+  case NV:    \
+    return launch_mean_batch_##NV(kk, cplx, half, jobs, queue, nwork, ws, grid, s);
     C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
   }
@@ -267,7 +308,33 @@ static KArgs base_args(const msig_job* j, const void* x, const void* y, const fl
   return a;
 }
 
+static int log2i(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
 }  // namespace msig
+
+// Opaque batch plan: device copies of the per-job descriptors and the work
+// queues, and the launch list of one run.
+struct msig_batch {
+  struct Class {
+    int N, kk;
+    bool cplx, half;
+    int32_t qoff, nwork;
+    int grid;
+  };
+  int device = 0;
+  std::vector<Class> classes;
+  msig::KArgs* d_jobs = nullptr;
+  msig::WorkItem* d_queue = nullptr;
+  msig::FinArgs* d_fin = nullptr;
+  msig::WorkItem* d_finq = nullptr;
+  int32_t nfin = 0;
+  int fin_grid = 1;
+  size_t ws_bytes = 0;
+};
 
 using namespace msig;
 
@@ -456,6 +523,172 @@ int msig_coherence_mean(const float* x, const float* y, int64_t rows, int64_t n,
                         void* stream) {
   msig_job j = op_job(MSIG_COH_TRIO_MEAN, rows, n, nperseg, hop, detrend);
   return msig_spectral_mean(&j, x, y, win, pxx, pyy, pxy, workspace, ws_bytes, stream);
+}
+
+}  // extern "C"
+
+static void free_plan(msig_batch* p) {
+  if (!p) return;
+  if (p->d_jobs) (void)hipFree(p->d_jobs);
+  if (p->d_queue) (void)hipFree(p->d_queue);
+  if (p->d_fin) (void)hipFree(p->d_fin);
+  if (p->d_finq) (void)hipFree(p->d_finq);
+  delete p;
+}
+
+template <class T>
+static int upload(const std::vector<T>& h, T** d, const char* what) {
+  if (h.empty()) return 0;
+  hipError_t e = hipMalloc(d, sizeof(T) * h.size());
+  if (e != hipSuccess) return hip_fail(e, what);
+  e = hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, what);
+  return 0;
+}
+
+extern "C" {
+
+int msig_batch_create(const msig_batch_job* jobs, int32_t njobs, const msig_tuning* tuning,
+                      msig_batch** plan) {
+  if (!plan) return fail(MSIG_EINVAL, "plan is NULL");
+  *plan = nullptr;
+  if (njobs < 1 || !jobs) return fail(MSIG_EINVAL, "a batch needs at least one job");
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
+  std::vector<KArgs> hk(njobs);
+  std::vector<FinArgs> hf(njobs);
+  std::map<std::tuple<int, int, bool, bool>, std::vector<WorkItem>> qs;
+  std::vector<WorkItem> finq;
+  size_t off = 0;  // floats
+  for (int32_t i = 0; i < njobs; ++i) {
+    const msig_batch_job& bj = jobs[i];
+    const msig_job* j = &bj.job;
+    int rc = check_job(j);
+    if (rc) return fail(rc, "job %d: %s", i, msig_last_error());
+    if (!is_mean(j->kind)) return fail(MSIG_EINVAL, "job %d: kind %d is not a mean kind", i, j->kind);
+    const bool pair = is_pair(j->kind);
+    if (!bj.x || !bj.win || !bj.out0 || (pair && !bj.y))
+      return fail(MSIG_EINVAL, "job %d: NULL input/output pointer", i);
+    if (j->kind == MSIG_COH_TRIO_MEAN && (!bj.out1 || !bj.out2))
+      return fail(MSIG_EINVAL, "job %d: coherence trio needs three outputs", i);
+    const bool cplx = j->flags & MSIG_F_COMPLEX_INPUT;
+    const int N = j->nfft;
+    const int l2 = log2i(N);
+    int maxrun = MEAN_MAX_RUN;
+    int64_t target = 1024;
+    if (tuning) {
+      if (tuning->max_run[l2] > 0) maxrun = tuning->max_run[l2] < MEAN_MAX_RUN ? tuning->max_run[l2] : MEAN_MAX_RUN;
+      if (tuning->target_blocks[l2] > 0) target = tuning->target_blocks[l2];
+    }
+    const Geo g = geometry(j->rows, j->nseg, N, cplx, maxrun, target);
+    const cf* tw = nullptr;
+    rc = twiddles(N, &tw);
+    if (rc) return rc;
+    KArgs a = base_args(j, bj.x, bj.y, bj.win, tw, g);
+    a.ws = nullptr;
+    a.ws_off = (int64_t)off;
+    hk[i] = a;
+    FinArgs f;
+    std::memset(&f, 0, sizeof f);
+    f.out0 = bj.out0;
+    f.out1 = bj.out1;
+    f.out2 = bj.out2;
+    f.rows = j->rows;
+    f.bpr = g.bpr;
+    f.N = N;
+    f.nbins = nbins_of(j);
+    f.nc = ncomp(j->kind);
+    f.kind = j->kind;
+    f.fold = cplx ? 0 : 1;
+    f.apply_factor =
+        (!cplx && (j->flags & MSIG_F_APPLY_FACTOR) && (j->flags & MSIG_F_ONESIDED)) ? 1 : 0;
+    f.scale = j->scale / (double)j->nseg;
+    f.ws_off = (int64_t)off;
+    hf[i] = f;
+    const int kk = j->kind == MSIG_PSD_MEAN ? K_PSD : (j->kind == MSIG_CSD_MEAN ? K_CSD : K_COH);
+    const bool half = !cplx && j->nperseg == N && 2 * j->nstep == N;
+    const int64_t nblk = j->rows * g.bpr;
+    if (nblk > INT32_MAX) return fail(MSIG_EINVAL, "job %d: too many blocks", i);
+    auto& q = qs[std::make_tuple(N, kk, cplx, half)];
+    for (int64_t b = 0; b < nblk; ++b) q.push_back(WorkItem{i, (int32_t)b});
+    const int64_t chunks = (f.nbins + FIN_BINS - 1) / FIN_BINS;
+    for (int64_t b = 0; b < j->rows * chunks; ++b) finq.push_back(WorkItem{i, (int32_t)b});
+    off += ((size_t)j->rows * g.bpr * f.nc * N + 63) / 64 * 64;
+  }
+  msig_batch* p = new msig_batch();
+  p->device = dev;
+  std::vector<WorkItem> queue;
+  for (auto& kv : qs) {
+    msig_batch::Class c;
+    c.N = std::get<0>(kv.first);
+    c.kk = std::get<1>(kv.first);
+    c.cplx = std::get<2>(kv.first);
+    c.half = std::get<3>(kv.first);
+    c.qoff = (int32_t)queue.size();
+    c.nwork = (int32_t)kv.second.size();
+    int bpc = 4;
+    if (tuning && tuning->blocks_per_cu[log2i(c.N)] > 0) bpc = tuning->blocks_per_cu[log2i(c.N)];
+    const int64_t cap = (int64_t)bpc * cus;
+    c.grid = (int)(c.nwork < cap ? c.nwork : cap);
+    queue.insert(queue.end(), kv.second.begin(), kv.second.end());
+    p->classes.push_back(c);
+  }
+  p->nfin = (int32_t)finq.size();
+  const int64_t fcap = (int64_t)8 * cus;
+  p->fin_grid = (int)(p->nfin < fcap ? p->nfin : fcap);
+  p->ws_bytes = off * sizeof(float);
+  int rc = upload(hk, &p->d_jobs, "upload job descriptors");
+  if (!rc) rc = upload(queue, &p->d_queue, "upload work queue");
+  if (!rc) rc = upload(hf, &p->d_fin, "upload finalize descriptors");
+  if (!rc) rc = upload(finq, &p->d_finq, "upload finalize queue");
+  if (rc) {
+    free_plan(p);
+    return rc;
+  }
+  *plan = p;
+  return 0;
+}
+
+int msig_batch_workspace_bytes(const msig_batch* plan, size_t* bytes) {
+  if (!plan || !bytes) return fail(MSIG_EINVAL, "NULL plan or bytes");
+  *bytes = plan->ws_bytes;
+  return 0;
+}
+
+int msig_batch_launches(const msig_batch* plan) {
+  if (!plan) return fail(MSIG_EINVAL, "NULL plan");
+  return (int)plan->classes.size() + 1;
+}
+
+int msig_batch_run(const msig_batch* plan, void* workspace, size_t ws_bytes, void* stream) {
+  if (!plan) return fail(MSIG_EINVAL, "NULL plan");
+  if (!workspace || ws_bytes < plan->ws_bytes)
+    return fail(MSIG_EINVAL, "workspace of %zu bytes needed (got %zu)", plan->ws_bytes, ws_bytes);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev != plan->device)
+    return fail(MSIG_EINVAL, "plan was built on device %d, current device is %d", plan->device, dev);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  float* ws = static_cast<float*>(workspace);
+  for (const auto& c : plan->classes) {
+    e = dispatch_mean_batch(c.N, c.kk, c.cplx, c.half, plan->d_jobs, plan->d_queue + c.qoff,
+                            c.nwork, ws, c.grid, s);
+    if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
+  }
+  hipLaunchKernelGGL(k_finalize_batch, dim3(plan->fin_grid), dim3(FIN_BINS * FIN_LANES), 0, s,
+                     plan->d_fin, plan->d_finq, plan->nfin, ws);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "batch finalize launch");
+  return 0;
+}
+
+int msig_batch_destroy(msig_batch* plan) {
+  free_plan(plan);
+  return 0;
 }
 
 int msig_mean_workspace_bytes(int32_t kind, int64_t rows, int64_t n, int32_t nperseg, int32_t hop,

@@ -153,7 +153,8 @@ __device__ __forceinline__ void prep_unit(cf* z, const float* w, int detrend, in
 // Grouped mean over segments: per-bin sums of |Zx|^2 (PSD), conj(Zx) Zy
 // (CSD) or both spectra and the cross term (COH), written per block.
 template <int N, int E, int KIND, bool CPLX, bool HALF>
-__device__ __forceinline__ void mean_body(const KArgs& a, cf* lds_all, float* red, const cf* twl) {
+__device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, cf* lds_all,
+                                          float* red, const cf* twl) {
   using PL = Plan<N, E>;
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
@@ -161,8 +162,6 @@ __device__ __forceinline__ void mean_body(const KArgs& a, cf* lds_all, float* re
   const int tid = threadIdx.x;
   const int g = tid / TPF, t = tid % TPF;
   cf* lds = lds_all + g * PL::LDS;
-  const int64_t row = blockIdx.x / a.bpr;
-  const int bir = blockIdx.x % a.bpr;
   const int unit0 = (bir * GPB + g) * a.run;
 
   using TH = TwiddleHome<N, E>;
@@ -290,7 +289,7 @@ __global__ __launch_bounds__(BT, (mean_waves<N, KIND>())) void k_mean(const KArg
     fill_twiddle_lds<N, E>(twl, a.twt);
     __syncthreads();
   }
-  mean_body<N, E, KIND, CPLX, HALF>(a, lds_all, red, twl);
+  mean_body<N, E, KIND, CPLX, HALF>(a, blockIdx.x / a.bpr, blockIdx.x % a.bpr, lds_all, red, twl);
 }
 
 // Per-frame spectra.  Real input: the two packed segments are separated with
@@ -315,7 +314,8 @@ __device__ __forceinline__ void separate(int t, const cf* z, cf* lds, cf* x0, cf
 }
 
 template <int N, int E, int OK, bool CPLX, bool HALF>
-__device__ __forceinline__ void frames_body(const KArgs& a, cf* lds_all, float* red, const cf* twl) {
+__device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir, cf* lds_all,
+                                            float* red, const cf* twl) {
   using PL = Plan<N, E>;
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
@@ -323,8 +323,6 @@ __device__ __forceinline__ void frames_body(const KArgs& a, cf* lds_all, float* 
   const int tid = threadIdx.x;
   const int g = tid / TPF, t = tid % TPF;
   cf* lds = lds_all + g * PL::LDS;
-  const int64_t row = blockIdx.x / a.bpr;
-  const int bir = blockIdx.x % a.bpr;
   const int unit0 = (bir * GPB + g) * a.run;
 
   using TH = TwiddleHome<N, E>;
@@ -453,7 +451,48 @@ __global__ __launch_bounds__(BT, (frames_waves<N, OK>())) void k_frames(const KA
     fill_twiddle_lds<N, E>(twl, a.twt);
     __syncthreads();
   }
-  frames_body<N, E, OK, CPLX, HALF>(a, lds_all, red, twl);
+  frames_body<N, E, OK, CPLX, HALF>(a, blockIdx.x / a.bpr, blockIdx.x % a.bpr, lds_all, red, twl);
+}
+
+
+// One entry of a ragged batch's work queue: block `blk` (row * bpr + bir) of
+// job `job`.  Entries of one FFT-size class are walked by a persistent grid.
+struct WorkItem {
+  int32_t job;
+  int32_t blk;
+};
+
+// Ragged batch of mean jobs that share (N, KIND, CPLX, HALF): a persistent
+// grid strides over the class's work queue; every entry runs exactly the
+// block that a standalone launch of its job would run, so each job's result
+// is the same alone and in the batch.
+template <int N, int E, int KIND, bool CPLX, bool HALF>
+__global__ __launch_bounds__(BT, (mean_waves<N, KIND>())) void k_mean_batch(
+    const KArgs* __restrict__ jobs, const WorkItem* __restrict__ queue, int32_t nwork,
+    float* ws_base) {
+  using PL = Plan<N, E>;
+  __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
+  __shared__ float red[2 * (BT / 64)];
+  __shared__ cf twl[TwiddleHome<N, E>::LDS_ELEMS];
+  bool filled = false;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem item = queue[w];
+    KArgs a = jobs[item.job];
+    a.ws = ws_base + a.ws_off;
+    if constexpr (TwiddleHome<N, E>::LDS) {
+      if (!filled) {  // every job of a class has the same N: fill once
+        fill_twiddle_lds<N, E>(twl, a.twt);
+        filled = true;
+      }
+    }
+    __syncthreads();
+    if constexpr (HALF) {
+      mean_body<N, E, KIND, false, true>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl);
+    } else {
+      mean_body<N, E, KIND, CPLX, false>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl);
+    }
+    __syncthreads();
+  }
 }
 
 }  // namespace msig

@@ -21,12 +21,13 @@ struct Geo {
 
 inline int elems_per_thread(int N) { return N < 16 ? N : 16; }
 
-inline Geo geometry(int64_t rows, int64_t nseg, int N, bool cplx, int maxrun) {
+inline Geo geometry(int64_t rows, int64_t nseg, int N, bool cplx, int maxrun,
+                    int64_t target_blocks = 1024) {
   Geo g;
   const int tpf = N / elems_per_thread(N);
   g.gpb = BT / tpf;
   g.units = (int32_t)(cplx ? nseg : (nseg + 1) / 2);
-  int64_t bprt = (1024 + rows - 1) / rows;
+  int64_t bprt = (target_blocks + rows - 1) / rows;
   if (bprt < 1) bprt = 1;
   int64_t run = (g.units + g.gpb * bprt - 1) / (g.gpb * bprt);
   if (run < 1) run = 1;
@@ -44,7 +45,10 @@ constexpr int FRAMES_MAX_RUN = 8;
   hipError_t launch_mean_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows,  \
                               hipStream_t s);                                              \
   hipError_t launch_frames_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows, \
-                                hipStream_t s);
+                                hipStream_t s);                                             \
+  hipError_t launch_mean_batch_##NV(int kind, bool cplx, bool half, const KArgs* jobs,      \
+                                    const WorkItem* queue, int32_t nwork, float* ws,        \
+                                    int grid, hipStream_t s);
 MSIG_DECLARE_LAUNCHERS(16)
 MSIG_DECLARE_LAUNCHERS(32)
 MSIG_DECLARE_LAUNCHERS(64)
@@ -64,6 +68,9 @@ MSIG_DECLARE_LAUNCHERS(4096)
   hipLaunchKernelGGL((k_mean<NV, (NV < 16 ? NV : 16), K, C, H>), grid, block, 0, s, a)
 #define MSIG_LAUNCH_FRAMES(NV, K, C, H) \
   hipLaunchKernelGGL((k_frames<NV, (NV < 16 ? NV : 16), K, C, H>), grid, block, 0, s, a)
+#define MSIG_LAUNCH_MEAN_BATCH(NV, K, C, H)                                          \
+  hipLaunchKernelGGL((k_mean_batch<NV, (NV < 16 ? NV : 16), K, C, H>), dim3(grid), dim3(BT), \
+                     0, s, jobs, queue, nwork, ws)
 #define MSIG_KIND_SWITCH(L, NV, K0, K1, K2, C, H) \
   if (kind == K0) L(NV, K0, C, H);                \
   else if (kind == K1) L(NV, K1, C, H);           \
@@ -95,6 +102,18 @@ MSIG_DECLARE_LAUNCHERS(4096)
     } else {                                                                               \
       if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES(NV, F_COMPLEX, false, false);              \
       else MSIG_KIND_SWITCH(MSIG_LAUNCH_FRAMES, NV, F_POWER, F_CSD, F_COH, false, false)   \
+    }                                                                                      \
+    return hipGetLastError();                                                              \
+  }                                                                                        \
+  hipError_t launch_mean_batch_##NV(int kind, bool cplx, bool half, const KArgs* jobs,      \
+                                    const WorkItem* queue, int32_t nwork, float* ws,        \
+                                    int grid, hipStream_t s) {                              \
+    if (cplx) {                                                                            \
+      MSIG_KIND_SWITCH(MSIG_LAUNCH_MEAN_BATCH, NV, K_PSD, K_CSD, K_COH, true, false)       \
+    } else if (half) {                                                                     \
+      MSIG_KIND_SWITCH(MSIG_LAUNCH_MEAN_BATCH, NV, K_PSD, K_CSD, K_COH, false, true)       \
+    } else {                                                                               \
+      MSIG_KIND_SWITCH(MSIG_LAUNCH_MEAN_BATCH, NV, K_PSD, K_CSD, K_COH, false, false)      \
     }                                                                                      \
     return hipGetLastError();                                                              \
   }                                                                                        \

@@ -25,6 +25,7 @@ struct KArgs {
   int32_t nbins;     // bins written per output row / frame
   int32_t apply_factor;
   float oscale;      // frames: power scale or sqrt(scale) for complex output
+  int64_t ws_off;    // batch plans: this job's partials start at ws_base + ws_off
 };
 
 // Raw samples of consecutive units of one row, fetched one unit ahead so the

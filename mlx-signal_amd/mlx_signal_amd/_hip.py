@@ -50,6 +50,28 @@ class MsigJob(ctypes.Structure):
     ]
 
 
+class MsigBatchJob(ctypes.Structure):
+    _fields_ = [
+        ("job", MsigJob),
+        ("x", ctypes.c_void_p),
+        ("y", ctypes.c_void_p),
+        ("win", ctypes.c_void_p),
+        ("out0", ctypes.c_void_p),
+        ("out1", ctypes.c_void_p),
+        ("out2", ctypes.c_void_p),
+    ]
+
+
+class MsigTuning(ctypes.Structure):
+    """Per FFT size (index log2(nfft)); 0 selects the library default."""
+
+    _fields_ = [
+        ("max_run", ctypes.c_int32 * 13),
+        ("target_blocks", ctypes.c_int32 * 13),
+        ("blocks_per_cu", ctypes.c_int32 * 13),
+    ]
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -73,6 +95,12 @@ SIGNATURES = {
     "msig_csd_mean": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _i32, _i32, _i32, _vp, _vp, _sz, _vp]),
     "msig_coherence_mean": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _sz, _vp]),
     "msig_mean_workspace_bytes": (ctypes.c_int, [_i32, _i64, _i64, _i32, _i32, ctypes.POINTER(_sz)]),
+    "msig_batch_create": (ctypes.c_int, [ctypes.POINTER(MsigBatchJob), _i32,
+                                         ctypes.POINTER(MsigTuning), ctypes.POINTER(_vp)]),
+    "msig_batch_workspace_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
+    "msig_batch_run": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
+    "msig_batch_launches": (ctypes.c_int, [_vp]),
+    "msig_batch_destroy": (ctypes.c_int, [_vp]),
 }
 
 _lib = None

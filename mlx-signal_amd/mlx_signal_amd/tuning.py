@@ -1,0 +1,133 @@
+"""Autotuner for batched and served spectral jobs.
+
+For each FFT size it times candidate launch shapes of the batch kernels on a
+representative ragged batch (several channel counts and lengths, welch and
+csd), with HIP events on the device, and keeps the fastest.  The knobs are
+the tile shape of a job's decomposition:
+
+* ``target_blocks`` - blocks one job's geometry aims for (how finely a job's
+  segments are split across blocks);
+* ``max_run``       - segment pairs one thread group sums before its partial
+  is written (run length of the register accumulation);
+* ``blocks_per_cu`` - resident blocks per CU of the persistent launch.
+
+The chosen table is cached per device name in a JSON file so later processes
+reuse it.  Jobs are planned with their size's tuned knobs both alone and in a
+batch, so tuning never breaks the alone-equals-in-mix property.
+"""
+
+from __future__ import annotations
+
+import itertools
+import json
+import os
+import statistics
+from pathlib import Path
+
+import torch
+
+from .batch import SpectralBatch, SpectralRequest
+
+__all__ = ["autotune", "cached_tuning", "load_tuning", "save_tuning", "CANDIDATES"]
+
+CANDIDATES = [
+    dict(target_blocks=tb, max_run=mr, blocks_per_cu=bpc)
+    for tb, mr, bpc in itertools.product((256, 1024, 4096), (8, 32), (2, 4, 8))
+]
+
+
+def _cache_path() -> Path:
+    root = os.environ.get("MSIG_TUNING_CACHE")
+    if root:
+        return Path(root)
+    return Path(os.path.expanduser("~/.cache/mlx_signal_amd/tuning.json"))
+
+
+def _device_key(device) -> str:
+    props = torch.cuda.get_device_properties(device)
+    return f"{props.name}|{getattr(props, 'gcnArchName', '')}|{props.multi_processor_count}"
+
+
+def load_tuning(device=None):
+    p = _cache_path()
+    if not p.exists():
+        return None
+    try:
+        table = json.loads(p.read_text()).get(_device_key(device or torch.cuda.current_device()))
+    except (ValueError, OSError):
+        return None
+    if not table:
+        return None
+    return {int(k): v for k, v in table.items()}
+
+
+def save_tuning(tuning, device=None) -> Path:
+    p = _cache_path()
+    p.parent.mkdir(parents=True, exist_ok=True)
+    data = {}
+    if p.exists():
+        try:
+            data = json.loads(p.read_text())
+        except (ValueError, OSError):
+            data = {}
+    data[_device_key(device or torch.cuda.current_device())] = {str(k): v for k, v in tuning.items()}
+    p.write_text(json.dumps(data, indent=1) + "\n")
+    return p
+
+
+def _probe_requests(nfft, dev, gen):
+    reqs = []
+    for ch, n, kind in ((1, 1 << 14, "welch"), (8, 1 << 17, "csd"), (64, 1 << 18, "welch"),
+                        (4, 1 << 20, "welch"), (32, 1 << 15, "csd")):
+        n = max(n, 2 * nfft)
+        x = torch.randn(ch, n, device=dev, generator=gen)
+        y = torch.randn(ch, n, device=dev, generator=gen) if kind == "csd" else None
+        reqs.append(SpectralRequest(kind, x, y, nperseg=nfft))
+    return reqs
+
+
+def _time_batch(b: SpectralBatch, reps: int) -> float:
+    b.run()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.run()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return statistics.median(ts)
+
+
+def autotune(nffts=(256, 1024, 4096), reps=5, seed=0, candidates=None, save=True):
+    """Time every candidate for every nfft; return {nfft: best knobs}."""
+    dev = torch.device("cuda", torch.cuda.current_device())
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    best = {}
+    report = {}
+    for nfft in nffts:
+        reqs = _probe_requests(nfft, dev, gen)
+        timings = []
+        for cand in candidates or CANDIDATES:
+            b = SpectralBatch(reqs, tuning={nfft: cand})
+            timings.append((_time_batch(b, reps), cand))
+            b.close()
+        timings.sort(key=lambda tc: tc[0])
+        best[nfft] = dict(timings[0][1])
+        report[nfft] = [(round(t, 4), c) for t, c in timings[:3]]
+        del reqs
+    if save:
+        save_tuning(best, dev)
+    autotune.last_report = report
+    return best
+
+
+def cached_tuning(nffts=(256, 1024, 4096)):
+    """The cached table for this device, tuning (and caching) it if absent."""
+    t = load_tuning()
+    if t is None or any(n not in t for n in nffts):
+        t = autotune(nffts)
+    return t
