@@ -154,7 +154,7 @@ __device__ __forceinline__ void prep_unit(cf* z, const float* w, int detrend, in
 // (CSD) or both spectra and the cross term (COH), written per block.
 template <int N, int E, int KIND, bool CPLX, bool HALF>
 __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, cf* lds_all,
-                                          float* red, const cf* twl) {
+                                          float* red, const cf* twl, float* ws) {
   using PL = Plan<N, E>;
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
@@ -247,7 +247,7 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
 
   // block reduction: groups summed in a fixed order
   float* redf = reinterpret_cast<float*>(lds_all);
-  float* dst = a.ws + ((size_t)row * a.bpr + bir) * NC * N;
+  float* dst = ws + ((size_t)row * a.bpr + bir) * NC * N;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     __syncthreads();
@@ -289,7 +289,8 @@ __global__ __launch_bounds__(BT, (mean_waves<N, KIND>())) void k_mean(const KArg
     fill_twiddle_lds<N, E>(twl, a.twt);
     __syncthreads();
   }
-  mean_body<N, E, KIND, CPLX, HALF>(a, blockIdx.x / a.bpr, blockIdx.x % a.bpr, lds_all, red, twl);
+  mean_body<N, E, KIND, CPLX, HALF>(a, blockIdx.x / a.bpr, blockIdx.x % a.bpr, lds_all, red, twl,
+                                    a.ws);
 }
 
 // Per-frame spectra.  Real input: the two packed segments are separated with
@@ -466,8 +467,15 @@ struct WorkItem {
 // grid strides over the class's work queue; every entry runs exactly the
 // block that a standalone launch of its job would run, so each job's result
 // is the same alone and in the batch.
+// The batch kernels keep the per-job descriptor and queue state live across
+// the job loop; one wave fewer than the standalone kernels avoids spills.
+template <int N, int KIND>
+constexpr int batch_waves() {
+  return mean_waves<N, KIND>() > 1 ? mean_waves<N, KIND>() - 1 : 1;
+}
+
 template <int N, int E, int KIND, bool CPLX, bool HALF>
-__global__ __launch_bounds__(BT, (mean_waves<N, KIND>())) void k_mean_batch(
+__global__ __launch_bounds__(BT, (batch_waves<N, KIND>())) void k_mean_batch(
     const KArgs* __restrict__ jobs, const WorkItem* __restrict__ queue, int32_t nwork,
     float* ws_base) {
   using PL = Plan<N, E>;
@@ -477,8 +485,8 @@ __global__ __launch_bounds__(BT, (mean_waves<N, KIND>())) void k_mean_batch(
   bool filled = false;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const WorkItem item = queue[w];
-    KArgs a = jobs[item.job];
-    a.ws = ws_base + a.ws_off;
+    const KArgs& a = jobs[item.job];
+    float* ws = ws_base + a.ws_off;
     if constexpr (TwiddleHome<N, E>::LDS) {
       if (!filled) {  // every job of a class has the same N: fill once
         fill_twiddle_lds<N, E>(twl, a.twt);
@@ -487,9 +495,11 @@ __global__ __launch_bounds__(BT, (mean_waves<N, KIND>())) void k_mean_batch(
     }
     __syncthreads();
     if constexpr (HALF) {
-      mean_body<N, E, KIND, false, true>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl);
+      mean_body<N, E, KIND, false, true>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl,
+                                         ws);
     } else {
-      mean_body<N, E, KIND, CPLX, false>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl);
+      mean_body<N, E, KIND, CPLX, false>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl,
+                                         ws);
     }
     __syncthreads();
   }
