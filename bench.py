@@ -211,6 +211,115 @@ def bench_ragged(torch, msig, dev, reps=3):
     return out
 
 
+def serving_schedule(torch, dev, n_req=2000, rate=2000.0, seed=11):
+    """Poisson(rate) arrivals of welch/csd requests: channels U{1..256}, lengths
+    log-uniform in [2^14, 2^22], nperseg in {256, 512, 1024, 4096}; payloads are
+    slices of a device-resident pool of seeded N(0,1) float32 samples."""
+    from mlx_signal_amd.batch import SpectralRequest
+
+    rng = np.random.default_rng(seed)
+    pool_n = 1 << 31  # 8 GiB
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    pool = torch.randn(pool_n, device=dev, generator=g)
+    t = 0.0
+    sched = []
+    for _ in range(n_req):
+        t += float(rng.exponential(1.0 / rate))
+        ch = int(rng.integers(1, 257))
+        n = int(2 ** rng.uniform(14, 22))
+        nps = int(rng.choice([256, 512, 1024, 4096]))
+        kind = str(rng.choice(["welch", "csd"]))
+        size = ch * n
+        ox = int(rng.integers(0, pool_n - size))
+        x = pool[ox : ox + size].view(ch, n)
+        y = None
+        if kind == "csd":
+            oy = int(rng.integers(0, pool_n - size))
+            y = pool[oy : oy + size].view(ch, n)
+        sched.append((t, SpectralRequest(kind, x, y, fs=FS, nperseg=nps)))
+    return pool, sched
+
+
+def _plain_serve(torch, msig, sched):
+    """One plain call per request, in arrival order, on one stream."""
+    t0 = time.perf_counter()
+    pending = list(sched)
+    inflight = []
+    done = []
+    i = 0
+    while i < len(pending) or inflight:
+        now = time.perf_counter() - t0
+        while i < len(pending) and pending[i][0] <= now:
+            ta, r = pending[i]
+            kw = dict(fs=r.fs, window=r.window, nperseg=r.nperseg)
+            if r.kind == "welch":
+                msig.welch(r.x, **kw)
+            else:
+                msig.csd(r.x, r.y, **kw)
+            ev = torch.cuda.Event()
+            ev.record()
+            inflight.append((ta, ev))
+            i += 1
+        still = []
+        for ta, ev in inflight:
+            if ev.query():
+                done.append((ta, time.perf_counter() - t0))
+            else:
+                still.append((ta, ev))
+        inflight = still
+    return done
+
+
+def _stats(done):
+    lat = sorted(d - a for a, d in done)
+    span = max(d for _, d in done) - min(a for a, _ in done)
+    pick = lambda p: lat[min(len(lat) - 1, int(round(p / 100 * (len(lat) - 1))))]
+    return {"requests": len(lat), "p50_ms": pick(50) * 1e3, "p99_ms": pick(99) * 1e3,
+            "throughput_req_s": len(lat) / span}
+
+
+def bench_serving(torch, msig, dev):
+    """Configuration 5: Poisson 2000 req/s; continuous batching vs one plain
+    call per request, plus saturation throughput (all requests at t = 0)."""
+    from mlx_signal_amd.serving import SpectralServer, latency_stats
+    from mlx_signal_amd.tuning import cached_tuning
+
+    pool, sched = serving_schedule(torch, dev)
+    torch.cuda.synchronize()
+    tuning = cached_tuning((256, 512, 1024, 4096))
+    nbytes = sum(4 * r.x.numel() * (2 if r.y is not None else 1) for _, r in sched)
+    nsamp = sum(r.x.numel() for _, r in sched)
+    # warm both paths (code objects, cached windows / grids)
+    warm = [(0.0, r) for _, r in sched[:8]]
+    SpectralServer(tuning=tuning).serve(warm)
+    _plain_serve(torch, msig, warm)
+    torch.cuda.synchronize()
+    out = {"requests": len(sched), "offered_req_s": 2000.0, "samples": nsamp,
+           "input_bytes": nbytes, "offered_GBps": nbytes / sched[-1][0] / 1e9}
+    srv = SpectralServer(hbm_budget=96 << 30, streams=3, max_inflight=3, tuning=tuning)
+    out["batched"] = latency_stats(srv.serve(sched))
+    out["batched"]["batches"] = srv.batches
+    torch.cuda.synchronize()
+    out["plain"] = _stats(_plain_serve(torch, msig, sched))
+    torch.cuda.synchronize()
+    burst = [(0.0, r) for _, r in sched]
+    t0 = time.perf_counter()
+    SpectralServer(hbm_budget=96 << 30, streams=3, max_inflight=3, tuning=tuning).serve(burst)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _plain_serve(torch, msig, burst)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    out["saturation"] = {"batched_s": tb, "plain_s": tp, "speedup": tp / tb,
+                         "batched_GBps": nbytes / tb / 1e9, "plain_GBps": nbytes / tp / 1e9,
+                         "batched_samples_per_s": nsamp / tb}
+    del pool, sched, burst
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,6 +410,7 @@ def main():
         secondary["welch_e2e_numpy_ms"] = statistics.median(ts) * 1e3
         del xs16
         secondary["ragged_512"] = bench_ragged(torch, msig, dev)
+        secondary["serving_2000rps"] = bench_serving(torch, msig, dev)
 
     out = {
         "metric": METRIC,

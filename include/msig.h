@@ -180,11 +180,22 @@ typedef struct msig_tuning {
 
 typedef struct msig_batch msig_batch; /* opaque plan */
 
+/* Host-only planning: no device allocation and no copy. */
 int msig_batch_create(const msig_batch_job* jobs, int32_t njobs, const msig_tuning* tuning,
                       msig_batch** plan);
+/* Bytes of the plan's device descriptor block (job descriptors + work queues). */
+int msig_batch_descriptor_bytes(const msig_batch* plan, size_t* bytes);
+/* Write the descriptor block into host memory `host` (>= descriptor bytes),
+ * laid out for the device address `device` it will be copied to.  The caller
+ * copies it (e.g. hipMemcpyAsync from pinned memory on the run's stream)
+ * before the first run that uses `device`. */
+int msig_batch_pack(const msig_batch* plan, void* host, size_t bytes, void* device);
 int msig_batch_workspace_bytes(const msig_batch* plan, size_t* bytes);
-/* Launches on `stream`; allocates nothing and copies nothing from the host. */
-int msig_batch_run(const msig_batch* plan, void* workspace, size_t ws_bytes, void* stream);
+/* Launches on `stream` using the descriptor block at `device` (packed for that
+ * address); allocates nothing and copies nothing from the host, so it can be
+ * captured in a hipGraph. */
+int msig_batch_run(const msig_batch* plan, const void* device, void* workspace, size_t ws_bytes,
+                   void* stream);
 /* Number of kernel launches one run makes (size classes + 1). */
 int msig_batch_launches(const msig_batch* plan);
 int msig_batch_destroy(msig_batch* plan);

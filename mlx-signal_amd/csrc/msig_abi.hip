@@ -316,8 +316,8 @@ static int log2i(int v) {
 
 }  // namespace msig
 
-// Opaque batch plan: device copies of the per-job descriptors and the work
-// queues, and the launch list of one run.
+// Opaque batch plan: the per-job descriptors and work queues on the host,
+// and the launch list of one run.  The device copy lives in caller memory.
 struct msig_batch {
   struct Class {
     int N, kk;
@@ -327,13 +327,14 @@ struct msig_batch {
   };
   int device = 0;
   std::vector<Class> classes;
-  msig::KArgs* d_jobs = nullptr;
-  msig::WorkItem* d_queue = nullptr;
-  msig::FinArgs* d_fin = nullptr;
-  msig::WorkItem* d_finq = nullptr;
-  int32_t nfin = 0;
+  std::vector<msig::KArgs> jobs;
+  std::vector<msig::WorkItem> queue;
+  std::vector<msig::FinArgs> fins;
+  std::vector<msig::WorkItem> finq;
   int fin_grid = 1;
   size_t ws_bytes = 0;
+  // byte offsets of the four arrays inside the descriptor block
+  size_t off_jobs = 0, off_queue = 0, off_fins = 0, off_finq = 0, desc_bytes = 0;
 };
 
 using namespace msig;
@@ -527,24 +528,9 @@ int msig_coherence_mean(const float* x, const float* y, int64_t rows, int64_t n,
 
 }  // extern "C"
 
-static void free_plan(msig_batch* p) {
-  if (!p) return;
-  if (p->d_jobs) (void)hipFree(p->d_jobs);
-  if (p->d_queue) (void)hipFree(p->d_queue);
-  if (p->d_fin) (void)hipFree(p->d_fin);
-  if (p->d_finq) (void)hipFree(p->d_finq);
-  delete p;
-}
+static void free_plan(msig_batch* p) { delete p; }
 
-template <class T>
-static int upload(const std::vector<T>& h, T** d, const char* what) {
-  if (h.empty()) return 0;
-  hipError_t e = hipMalloc(d, sizeof(T) * h.size());
-  if (e != hipSuccess) return hip_fail(e, what);
-  e = hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
-  if (e != hipSuccess) return hip_fail(e, what);
-  return 0;
-}
+static size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
 extern "C" {
 
@@ -620,35 +606,53 @@ int msig_batch_create(const msig_batch_job* jobs, int32_t njobs, const msig_tuni
   }
   msig_batch* p = new msig_batch();
   p->device = dev;
-  std::vector<WorkItem> queue;
+  p->jobs = std::move(hk);
+  p->fins = std::move(hf);
   for (auto& kv : qs) {
     msig_batch::Class c;
     c.N = std::get<0>(kv.first);
     c.kk = std::get<1>(kv.first);
     c.cplx = std::get<2>(kv.first);
     c.half = std::get<3>(kv.first);
-    c.qoff = (int32_t)queue.size();
+    c.qoff = (int32_t)p->queue.size();
     c.nwork = (int32_t)kv.second.size();
     int bpc = 4;
     if (tuning && tuning->blocks_per_cu[log2i(c.N)] > 0) bpc = tuning->blocks_per_cu[log2i(c.N)];
     const int64_t cap = (int64_t)bpc * cus;
     c.grid = (int)(c.nwork < cap ? c.nwork : cap);
-    queue.insert(queue.end(), kv.second.begin(), kv.second.end());
+    p->queue.insert(p->queue.end(), kv.second.begin(), kv.second.end());
     p->classes.push_back(c);
   }
-  p->nfin = (int32_t)finq.size();
+  p->finq = std::move(finq);
+  const int64_t nfin = (int64_t)p->finq.size();
   const int64_t fcap = (int64_t)8 * cus;
-  p->fin_grid = (int)(p->nfin < fcap ? p->nfin : fcap);
+  p->fin_grid = (int)(nfin < fcap ? nfin : fcap);
   p->ws_bytes = off * sizeof(float);
-  int rc = upload(hk, &p->d_jobs, "upload job descriptors");
-  if (!rc) rc = upload(queue, &p->d_queue, "upload work queue");
-  if (!rc) rc = upload(hf, &p->d_fin, "upload finalize descriptors");
-  if (!rc) rc = upload(finq, &p->d_finq, "upload finalize queue");
-  if (rc) {
-    free_plan(p);
-    return rc;
-  }
+  p->off_jobs = 0;
+  p->off_queue = align256(p->off_jobs + sizeof(KArgs) * p->jobs.size());
+  p->off_fins = align256(p->off_queue + sizeof(WorkItem) * p->queue.size());
+  p->off_finq = align256(p->off_fins + sizeof(FinArgs) * p->fins.size());
+  p->desc_bytes = align256(p->off_finq + sizeof(WorkItem) * p->finq.size());
   *plan = p;
+  return 0;
+}
+
+int msig_batch_descriptor_bytes(const msig_batch* plan, size_t* bytes) {
+  if (!plan || !bytes) return fail(MSIG_EINVAL, "NULL plan or bytes");
+  *bytes = plan->desc_bytes;
+  return 0;
+}
+
+int msig_batch_pack(const msig_batch* plan, void* host, size_t bytes, void* device) {
+  if (!plan || !host || !device) return fail(MSIG_EINVAL, "NULL plan or buffer");
+  if (bytes < plan->desc_bytes)
+    return fail(MSIG_EINVAL, "descriptor block of %zu bytes needed (got %zu)", plan->desc_bytes, bytes);
+  char* h = static_cast<char*>(host);
+  std::memcpy(h + plan->off_jobs, plan->jobs.data(), sizeof(KArgs) * plan->jobs.size());
+  std::memcpy(h + plan->off_queue, plan->queue.data(), sizeof(WorkItem) * plan->queue.size());
+  std::memcpy(h + plan->off_fins, plan->fins.data(), sizeof(FinArgs) * plan->fins.size());
+  std::memcpy(h + plan->off_finq, plan->finq.data(), sizeof(WorkItem) * plan->finq.size());
+  (void)device;  // the block holds no absolute pointers into itself
   return 0;
 }
 
@@ -663,8 +667,9 @@ int msig_batch_launches(const msig_batch* plan) {
   return (int)plan->classes.size() + 1;
 }
 
-int msig_batch_run(const msig_batch* plan, void* workspace, size_t ws_bytes, void* stream) {
-  if (!plan) return fail(MSIG_EINVAL, "NULL plan");
+int msig_batch_run(const msig_batch* plan, const void* device, void* workspace, size_t ws_bytes,
+                   void* stream) {
+  if (!plan || !device) return fail(MSIG_EINVAL, "NULL plan or descriptor block");
   if (!workspace || ws_bytes < plan->ws_bytes)
     return fail(MSIG_EINVAL, "workspace of %zu bytes needed (got %zu)", plan->ws_bytes, ws_bytes);
   int dev = 0;
@@ -673,14 +678,18 @@ int msig_batch_run(const msig_batch* plan, void* workspace, size_t ws_bytes, voi
   if (dev != plan->device)
     return fail(MSIG_EINVAL, "plan was built on device %d, current device is %d", plan->device, dev);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const char* d = static_cast<const char*>(device);
+  const KArgs* jobs = reinterpret_cast<const KArgs*>(d + plan->off_jobs);
+  const WorkItem* queue = reinterpret_cast<const WorkItem*>(d + plan->off_queue);
+  const FinArgs* fins = reinterpret_cast<const FinArgs*>(d + plan->off_fins);
+  const WorkItem* finq = reinterpret_cast<const WorkItem*>(d + plan->off_finq);
   float* ws = static_cast<float*>(workspace);
   for (const auto& c : plan->classes) {
-    e = dispatch_mean_batch(c.N, c.kk, c.cplx, c.half, plan->d_jobs, plan->d_queue + c.qoff,
-                            c.nwork, ws, c.grid, s);
+    e = dispatch_mean_batch(c.N, c.kk, c.cplx, c.half, jobs, queue + c.qoff, c.nwork, ws, c.grid, s);
     if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
   }
-  hipLaunchKernelGGL(k_finalize_batch, dim3(plan->fin_grid), dim3(FIN_BINS * FIN_LANES), 0, s,
-                     plan->d_fin, plan->d_finq, plan->nfin, ws);
+  hipLaunchKernelGGL(k_finalize_batch, dim3(plan->fin_grid), dim3(FIN_BINS * FIN_LANES), 0, s, fins,
+                     finq, (int32_t)plan->finq.size(), ws);
   e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "batch finalize launch");
   return 0;

@@ -25,7 +25,26 @@
 #define MSIG_HD inline
 #endif
 
+// Global-address-space qualifier for device code.  Pointers that reach a
+// kernel through memory (batch job descriptors) are generic to the compiler,
+// which then emits FLAT accesses; casting them to this space restores
+// global_load / global_store.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MSIG_GLOBAL __attribute__((address_space(1)))
+#else
+#define MSIG_GLOBAL
+#endif
+
 namespace msig {
+
+template <class T>
+MSIG_HD const MSIG_GLOBAL T* as_global(const T* p) {
+  return (const MSIG_GLOBAL T*)p;
+}
+template <class T>
+MSIG_HD MSIG_GLOBAL T* as_global_mut(T* p) {
+  return (MSIG_GLOBAL T*)p;
+}
 
 struct cf {
   float re, im;
@@ -208,8 +227,8 @@ struct Plan {
 
 // Load the per-thread twiddle registers from an N-entry table of
 // exp(-2*pi*i*e/N) (computed in FP64 by the host, stored as float pairs).
-template <int N, int E, int P = 1>
-MSIG_HD void load_twiddles(int t, const cf* table, cf* tw) {
+template <int N, int E, int P = 1, class TablePtr>
+MSIG_HD void load_twiddles(int t, TablePtr table, cf* tw) {
   using PL = Plan<N, E>;
   if constexpr (P < PL::NP) {
     constexpr int R = PL::radix(P);
@@ -218,7 +237,10 @@ MSIG_HD void load_twiddles(int t, const cf* table, cf* tw) {
     for (int u = 0; u < E / R; ++u)
 #pragma unroll
       for (int j = 1; j < R; ++j)
-        tw[off + u * (R - 1) + (j - 1)] = table[PL::template tw_exp<P>(t, u * R + j)];
+      {
+        const int ex = PL::template tw_exp<P>(t, u * R + j);
+        tw[off + u * (R - 1) + (j - 1)] = cf{table[ex].re, table[ex].im};
+      }
     load_twiddles<N, E, P + 1>(t, table, tw);
   }
 }

@@ -98,7 +98,7 @@ __device__ __forceinline__ void fill_twiddle_lds(cf* twl, const cf* table) {
   using PL = Plan<N, E>;
   for (int t = threadIdx.x; t < PL::TPF; t += BT) {
     cf tmp[PL::TWN];
-    load_twiddles<N, E>(t, table, tmp);
+    load_twiddles<N, E>(t, as_global(table), tmp);
 #pragma unroll
     for (int s = 0; s < PL::TWN; ++s) twl[s * PL::TPF + t] = tmp[s];
   }
@@ -170,14 +170,14 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
   if constexpr (TH::LDS) {
     tw = twl + t;
   } else {
-    load_twiddles<N, E>(t, a.twt, twr);
+    load_twiddles<N, E>(t, as_global(a.twt), twr);
     tw = twr;
   }
   float w[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int n = t + TPF * e;
-    w[e] = n < a.nperseg ? a.win[n] : 0.f;
+    w[e] = n < a.nperseg ? as_global(a.win)[n] : 0.f;
   }
   const size_t esz = CPLX ? sizeof(cf) : sizeof(float);
   UnitStream<N, E, CPLX, HALF> sx, sy;
@@ -247,7 +247,7 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
 
   // block reduction: groups summed in a fixed order
   float* redf = reinterpret_cast<float*>(lds_all);
-  float* dst = ws + ((size_t)row * a.bpr + bir) * NC * N;
+  MSIG_GLOBAL float* dst = as_global_mut(ws) + ((size_t)row * a.bpr + bir) * NC * N;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     __syncthreads();
@@ -332,14 +332,14 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
   if constexpr (TH::LDS) {
     tw = twl + t;
   } else {
-    load_twiddles<N, E>(t, a.twt, twr);
+    load_twiddles<N, E>(t, as_global(a.twt), twr);
     tw = twr;
   }
   float w[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int n = t + TPF * e;
-    w[e] = n < a.nperseg ? a.win[n] : 0.f;
+    w[e] = n < a.nperseg ? as_global(a.win)[n] : 0.f;
   }
   const size_t esz = CPLX ? sizeof(cf) : sizeof(float);
   UnitStream<N, E, CPLX, HALF> sx, sy;
@@ -471,7 +471,7 @@ struct WorkItem {
 // the job loop; one wave fewer than the standalone kernels avoids spills.
 template <int N, int KIND>
 constexpr int batch_waves() {
-  return mean_waves<N, KIND>() > 1 ? mean_waves<N, KIND>() - 1 : 1;
+  return mean_waves<N, KIND>();
 }
 
 template <int N, int E, int KIND, bool CPLX, bool HALF>

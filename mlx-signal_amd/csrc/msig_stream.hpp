@@ -49,7 +49,7 @@ struct UnitStream {
   float h0[HALF ? H : 1], h1[HALF ? H : 1], h2[HALF ? H : 1];
 
   MSIG_HD void half_load(float* dst, int64_t h, bool valid) {
-    const float* p = reinterpret_cast<const float*>(row) + h * (N / 2);
+    const MSIG_GLOBAL float* p = as_global(reinterpret_cast<const float*>(row)) + h * (N / 2);
     if (valid) {
 #pragma unroll
       for (int e = 0; e < H; ++e) dst[e] = p[t + TPF * e];
@@ -75,11 +75,11 @@ struct UnitStream {
       half_load(h1, s0 + 1, active);
       half_load(h2, s0 + 2, v1);
     } else if constexpr (!CPLX) {
-      const float* xrow = reinterpret_cast<const float*>(row);
+      const MSIG_GLOBAL float* xrow = as_global(reinterpret_cast<const float*>(row));
       const int64_t s0 = 2 * (int64_t)unit;
       const bool v1 = active && (s0 + 1 < a.nseg);
-      const float* p0 = xrow + s0 * a.hop;
-      const float* p1 = p0 + a.hop;
+      const MSIG_GLOBAL float* p0 = xrow + s0 * a.hop;
+      const MSIG_GLOBAL float* p1 = p0 + a.hop;
       if (a.nperseg == N && v1) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -96,11 +96,16 @@ struct UnitStream {
         }
       }
     } else {
-      const cf* p0 = reinterpret_cast<const cf*>(row) + (int64_t)unit * a.hop;
+      const MSIG_GLOBAL cf* p0 = as_global(reinterpret_cast<const cf*>(row)) + (int64_t)unit * a.hop;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int n = t + TPF * e;
-        raw[e] = (active && n < a.nperseg) ? p0[n] : cf{0.f, 0.f};
+        if (active && n < a.nperseg) {
+          raw[e].re = p0[n].re;
+          raw[e].im = p0[n].im;
+        } else {
+          raw[e] = cf{0.f, 0.f};
+        }
       }
     }
   }

@@ -13,7 +13,10 @@ from pathlib import Path
 
 import torch
 
-LIB_PATH = Path(__file__).with_name("libmsig.so")
+import os
+
+# MSIG_LIBRARY selects another build of the library (e.g. a tuning variant)
+LIB_PATH = Path(os.environ.get("MSIG_LIBRARY") or Path(__file__).with_name("libmsig.so"))
 ABI_VERSION = 1
 
 MSIG_EINVAL = -22
@@ -97,8 +100,10 @@ SIGNATURES = {
     "msig_mean_workspace_bytes": (ctypes.c_int, [_i32, _i64, _i64, _i32, _i32, ctypes.POINTER(_sz)]),
     "msig_batch_create": (ctypes.c_int, [ctypes.POINTER(MsigBatchJob), _i32,
                                          ctypes.POINTER(MsigTuning), ctypes.POINTER(_vp)]),
+    "msig_batch_descriptor_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
+    "msig_batch_pack": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
     "msig_batch_workspace_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
-    "msig_batch_run": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
+    "msig_batch_run": (ctypes.c_int, [_vp, _vp, _vp, _sz, _vp]),
     "msig_batch_launches": (ctypes.c_int, [_vp]),
     "msig_batch_destroy": (ctypes.c_int, [_vp]),
 }

@@ -23,12 +23,11 @@ from __future__ import annotations
 import ctypes
 import dataclasses
 
-import numpy as np
 import torch
 
 from . import _hip
 from ._device import compute_device
-from .spectral import _triage_segments
+from .spectral import _freqs, _triage_segments, _window_dev
 
 __all__ = ["SpectralRequest", "SpectralBatch", "tuning_struct"]
 
@@ -93,6 +92,15 @@ class SpectralBatch:
         sz = ctypes.c_size_t(0)
         _hip.check(lib.msig_batch_workspace_bytes(handle, ctypes.byref(sz)), "batch workspace")
         self.workspace = torch.empty(max(1, sz.value), dtype=torch.uint8, device=self.device)
+        # descriptor block: packed into pinned host memory, copied to the
+        # device asynchronously on the current stream (no host-side sync)
+        _hip.check(lib.msig_batch_descriptor_bytes(handle, ctypes.byref(sz)), "batch descriptors")
+        self.descriptors = torch.empty(max(256, sz.value), dtype=torch.uint8, device=self.device)
+        host = torch.empty(self.descriptors.numel(), dtype=torch.uint8, pin_memory=True)
+        _hip.check(lib.msig_batch_pack(handle, _hip.ptr(host), host.numel(),
+                                       _hip.ptr(self.descriptors)), "batch pack")
+        self.descriptors.copy_(host, non_blocking=True)
+        self._host_desc = host  # pinned source must outlive the copy
         self.launches = lib.msig_batch_launches(handle)
 
     def _plan(self, r: SpectralRequest) -> _hip.MsigBatchJob:
@@ -137,11 +145,11 @@ class SpectralBatch:
         nseg = (n - nperseg) // nstep + 1
         dtype = torch.complex64 if r.kind == "csd" else torch.float32
         out = torch.empty(rows, bins, dtype=dtype, device=self.device)
-        win = torch.from_numpy(win_np.astype(np.float32)).to(self.device)
+        win = _window_dev(r.window, win_np, self.device)
         self._keep += [x2, y2, win, out]
         self.outputs.append(out.reshape(*batch_shape, bins))
-        freqs = np.fft.rfftfreq(nfft, 1 / r.fs) if onesided else np.fft.fftfreq(nfft, 1 / r.fs)
-        self.freqs.append(torch.from_numpy(freqs.astype(np.float32)).to(self.device))
+        # shared, read-only frequency grid (cached on the device per nfft / fs)
+        self.freqs.append(_freqs(nfft, r.fs, "onesided" if onesided else "twosided", self.device))
         self.samples += rows * n
         self.bytes += (2 if pair else 1) * 4 * rows * n + out.element_size() * rows * bins
         flags = (_hip.F_ONESIDED if onesided else 0) | (_hip.F_DETREND_CONST if const else 0)
@@ -156,8 +164,8 @@ class SpectralBatch:
         """Launch the batch on the current stream of its device."""
         lib = _hip.load()
         stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-        rc = lib.msig_batch_run(self._handle, _hip.ptr(self.workspace), self.workspace.numel(),
-                                stream)
+        rc = lib.msig_batch_run(self._handle, _hip.ptr(self.descriptors), _hip.ptr(self.workspace),
+                                self.workspace.numel(), stream)
         _hip.check(rc, "batch run")
         return self.outputs
 
