@@ -65,14 +65,20 @@ def make_inputs(torch, rows, n, seed, dev, with_y=True):
     return x.contiguous(), y.contiguous()
 
 
-def time_steps(torch, fn, bufs, steps, warmup):
+def time_steps(torch, fn, bufs, steps, warmup, kernel_timing=False):
     """Wall clock over K steps (synchronized both sides) plus per-step HIP event
-    durations on the current stream (the stream the kernels launch on)."""
+    durations on the current stream (the stream the kernels launch on).  With
+    kernel_timing, the library also records HIP events on that stream around
+    each step's fused transform kernel; returns its mean duration too."""
+    from mlx_signal_amd import _kernels
+
     for i in range(warmup):
         fn(bufs[i % len(bufs)])
     torch.cuda.synchronize()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(steps)]
+    if kernel_timing:
+        _kernels.kernel_timing_arm(steps)
     t0 = time.perf_counter()
     for i in range(steps):
         evs[i][0].record()
@@ -81,7 +87,12 @@ def time_steps(torch, fn, bufs, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ev_ms = [a.elapsed_time(b) for a, b in evs]
-    return wall / steps * 1e3, statistics.median(ev_ms), ev_ms
+    kernel_ms = None
+    if kernel_timing:
+        tot, n = _kernels.kernel_timing_read()
+        assert n == steps, f"timed {n} fused kernels over {steps} steps"
+        kernel_ms = tot / n
+    return wall / steps * 1e3, statistics.median(ev_ms), ev_ms, kernel_ms
 
 
 def traffic_from_profile():
@@ -120,6 +131,106 @@ def cpu_baseline():
                   f"(reference dispatch='scipy' route), median of 3: {t * 1e3:.1f} ms; "
                   f"host {os.cpu_count()} logical CPUs, 1 used",
     }
+
+
+def _cpu_welch_chunk(args):
+    import scipy.signal
+
+    xs, kw = args
+    return scipy.signal.welch(xs, **kw)[1]
+
+
+def cpu_all_cores():
+    """The same scipy route sharded by channel over every host core (one
+    process per core), 2^20-sample float32 channels, median of 3."""
+    import concurrent.futures
+    import multiprocessing
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = max(1, min(cores, 16))  # the host share of one GPU
+    rows = max(8, cores)
+    rng = np.random.default_rng(0)
+    xs = rng.standard_normal((rows, NS)).astype(np.float32)
+    kw = dict(fs=FS, window="hann", nperseg=NPERSEG, noverlap=NOVERLAP)
+    chunks = [(c, kw) for c in np.array_split(xs, cores)]
+    ctx = multiprocessing.get_context("fork")
+    with concurrent.futures.ProcessPoolExecutor(cores, mp_context=ctx) as ex:
+        list(ex.map(_cpu_welch_chunk, chunks))
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            list(ex.map(_cpu_welch_chunk, chunks))
+            ts.append(time.perf_counter() - t0)
+    t = statistics.median(ts)
+    return {"value": rows * NS / t, "unit": "samples/s", "cores": cores,
+            "sample": f"scipy.signal.welch float32 {rows}ch x 2^20 sharded by channel over "
+                      f"{cores} processes, median of 3: {t * 1e3:.1f} ms"}
+
+
+def bench_precision(torch, msig, dev, steps=20):
+    """Configuration 3: welch over 256 channels x 2^16 (N(0,1) plus one tone
+    per channel), nperseg 256, noverlap 128, hann: the FP32 kernels vs the
+    matrix-core FP16 / BF16 hi+lo forms, with max and RMS per-bin error
+    relative to each channel's FP64 peak (8 channels against the oracle)."""
+    import oracle
+    from mlx_signal_amd import _kernels
+
+    rows, n = 256, 1 << 16
+    kw = dict(fs=FS, window="hann", nperseg=256, noverlap=128)
+    bufs = []
+    for seed in range(3):
+        g = torch.Generator(device=dev)
+        g.manual_seed(100 + seed)
+        t = torch.arange(n, device=dev, dtype=torch.float32)
+        f0 = torch.rand(rows, 1, device=dev, generator=g) * 0.5
+        bufs.append(torch.randn(rows, n, device=dev, generator=g) + torch.sin(2 * np.pi * f0 * t))
+    ref = oracle.welch(bufs[0][:8].cpu().numpy(), **kw)[1]
+    peak = np.abs(ref).max(axis=-1, keepdims=True)
+    nbytes = welch_bytes(rows, n, 129)
+    out = {"workload": "welch 256ch x 2^16 f32, nperseg=256, noverlap=128, hann, density, mean",
+           "algorithmic_bytes": nbytes}
+    for precision in ("fp32", "fp16_split", "bf16_split"):
+        with msig.config_context(precision=precision):
+            def step(x):
+                return msig.welch(x, **kw)
+
+            w, e, _, k = time_steps(torch, step, bufs, steps, 3, kernel_timing=True)
+            p = step(bufs[0])[1][:8].cpu().numpy()
+        rel = np.abs(p - ref) / peak
+        out[precision] = {"ms_event": e, "ms_wall": w, "ms_kernel": k,
+                          "samples_per_s": rows * n / (e * 1e-3),
+                          "kernel_GBps": nbytes / (k * 1e-3) / 1e9,
+                          "kernel_roofline_frac": nbytes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "max_rel_err_vs_fp64_peak": float(rel.max()),
+                          "rms_rel_err_vs_fp64_peak": float(np.sqrt(np.mean(rel ** 2)))}
+    out["mfma_launches"] = {k: v for k, v in _kernels.launches.items() if "split" in k}
+    del bufs
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_small(torch, msig, dev, steps=50):
+    """Configuration 1: welch over 1 x 2^16 float32 N(0,1), nperseg 256,
+    noverlap 128 (launch-latency bound), beside scipy on one host core."""
+    import oracle
+
+    rng = np.random.default_rng(0)
+    xh = rng.standard_normal(1 << 16).astype(np.float32)
+    kw = dict(fs=FS, window="hann", nperseg=256, noverlap=128)
+    x = torch.from_numpy(xh).to(dev)
+    w, e, _, _ = time_steps(torch, lambda b: msig.welch(b, **kw), [x], steps, 5)
+    p = msig.welch(x, **kw)[1].cpu().numpy()
+    ref = oracle.welch(xh, **kw)[1]
+    oracle.scipy_route("welch", xh, **kw)
+    ts = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        oracle.scipy_route("welch", xh, **kw)
+        ts.append(time.perf_counter() - t0)
+    return {"workload": "welch 1ch x 2^16 f32, nperseg=256, noverlap=128, hann",
+            "ms_event": e, "ms_wall": w, "samples_per_s_event": (1 << 16) / (e * 1e-3),
+            "max_err_vs_fp64_peak": float(np.abs(p - ref).max() / np.abs(ref).max()),
+            "cpu_scipy_ms": statistics.median(ts) * 1e3}
 
 
 def ragged_requests(torch, dev, n_inst=512, seed=7):
@@ -330,6 +441,9 @@ def main():
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run 3 device copies of a 256 MiB buffer (a known-size kernel "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="run only the 256ch x 2^16 precision comparison (FP32 vs matrix-core "
+                         "FP16/BF16 split) and print its JSON; for profiling that kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -347,6 +461,9 @@ def main():
     torch.cuda.set_device(dev)
     _hip.load()
     msig.set_config(dispatch="gpu")
+    if args.precision_only:
+        print(json.dumps(bench_precision(torch, msig, dev, steps=args.steps)), flush=True)
+        return
 
     bins = NPERSEG // 2 + 1
     xs = [make_inputs(torch, C, NS, seed, dev, with_y=False)[0] for seed in range(3)]
@@ -355,9 +472,12 @@ def main():
     def step(x):
         return msig.welch(x, fs=FS, window="hann", nperseg=NPERSEG, noverlap=NOVERLAP)
 
-    ms_wall, ms_ev, ev_all = time_steps(torch, step, xs, args.steps, args.warmup)
+    ms_wall, ms_ev, ev_all, ms_kernel = time_steps(torch, step, xs, args.steps, args.warmup,
+                                                   kernel_timing=True)
     B = welch_bytes(C, NS, bins)
-    achieved = B / (ms_ev * 1e-3) / 1e9
+    # roofline of the dominant kernel (the fused transform + segment-sum
+    # kernel; the finalize kernel reads only the block partials)
+    achieved = B / (ms_kernel * 1e-3) / 1e9
 
     # accuracy gate on the measured data: 2 channels vs the FP64 oracle
     _, p = step(xs[0])
@@ -383,7 +503,7 @@ def main():
             ("csd", lambda b: msig.csd(b[0], b[1], **kw), 8 * C * NS + 8 * C * bins),
             ("coherence", lambda b: msig.coherence(b[0], b[1], **kw), 8 * C * NS + 4 * C * bins),
         ]:
-            w, e, _ = time_steps(torch, fn, pairs, max(5, args.steps // 2), 2)
+            w, e, _, _ = time_steps(torch, fn, pairs, max(5, args.steps // 2), 2)
             secondary[name] = {"ms_event": e, "ms_wall": w, "GBps": nbytes / (e * 1e-3) / 1e9,
                                "pairs_per_s": C * NS / (e * 1e-3)}
         del x2, y2, x3, y3, pairs
@@ -396,7 +516,7 @@ def main():
             ("spectrogram_16x2^20", lambda b: msig.spectrogram(b, nperseg=1024, noverlap=512),
              4 * 16 * NS + 4 * 16 * nseg_sp * bins),
         ]:
-            w, e, _ = time_steps(torch, fn, xs16, max(5, args.steps // 2), 2)
+            w, e, _, _ = time_steps(torch, fn, xs16, max(5, args.steps // 2), 2)
             secondary[name] = {"ms_event": e, "ms_wall": w, "GBps": nbytes / (e * 1e-3) / 1e9,
                                "samples_per_s": 16 * NS / (e * 1e-3)}
         # end-to-end: NumPy in -> tensor -> host out (the reference's "e2e" column)
@@ -409,6 +529,8 @@ def main():
             ts.append(time.perf_counter() - t0)
         secondary["welch_e2e_numpy_ms"] = statistics.median(ts) * 1e3
         del xs16
+        secondary["small_1x2^16"] = bench_small(torch, msig, dev)
+        secondary["precision_256x2^16"] = bench_precision(torch, msig, dev)
         secondary["ragged_512"] = bench_ragged(torch, msig, dev)
         secondary["serving_2000rps"] = bench_serving(torch, msig, dev)
 
@@ -430,7 +552,10 @@ def main():
                                "hann, detrend=constant, density, mean -> 513 bins",
                    "channels": C, "samples_per_channel": NS, "nperseg": NPERSEG,
                    "noverlap": NOVERLAP, "parallelism": "single GPU"},
+        "ms_dominant_kernel": ms_kernel,
+        "dominant_kernel": "k_mean<1024,16,PSD,real,half-hop> (fused window+FFT+|X|^2+segment sum)",
         "hbm_GBps": achieved,
+        "step_GBps": B / (ms_ev * 1e-3) / 1e9,
         "max_err_vs_fp64_peak": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile()},
@@ -438,6 +563,8 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        if not args.no_secondary:
+            secondary["cpu_all_cores"] = cpu_all_cores()
     print(json.dumps(out), flush=True)
 
 

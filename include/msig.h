@@ -64,6 +64,13 @@ enum {
 #define MSIG_F_ONESIDED 0x2      /* real input: emit bins 0..nfft/2 only           */
 #define MSIG_F_APPLY_FACTOR 0x4  /* multiply onesided bins 1..nfft/2-1 by 2        */
 #define MSIG_F_DETREND_CONST 0x8 /* subtract each segment's mean before windowing  */
+/* Precision form (MSIG_PSD_MEAN, real input, nperseg = nfft = 256 only, else
+ * MSIG_ENOTSUP): the two 16-point DFT stages run on the matrix cores with
+ * hi + lo split FP16 or BF16 operands and FP32 accumulation; the twiddle
+ * stage, detrend, window and the FP64 segment sum are as in the FP32 path.
+ * No counterpart in the reference (precision study). */
+#define MSIG_F_MFMA_F16 0x10
+#define MSIG_F_MFMA_BF16 0x20
 
 typedef struct msig_job {
   int32_t kind;    /* MSIG_* kind above                                         */
@@ -204,6 +211,17 @@ int msig_batch_destroy(msig_batch* plan);
  * (kind = MSIG_PSD_MEAN, MSIG_CSD_MEAN or MSIG_COH_TRIO_MEAN). */
 int msig_mean_workspace_bytes(int32_t kind, int64_t rows, int64_t n, int32_t nperseg, int32_t hop,
                               size_t* bytes);
+
+/* Kernel timing (instrumentation; no counterpart in the reference, which
+ * times whole calls).  After msig_kernel_timing_arm(k) on a device, the next
+ * k calls of msig_spectral_mean / msig_spectral_frames (and the operator-level
+ * entry points built on them) on that device record a HIP event pair on their
+ * stream directly around their fused transform kernel, excluding the
+ * finalize.  msig_kernel_timing_read waits for the recorded pairs, returns
+ * their summed elapsed milliseconds and count, and disarms.  k = 0 disarms.
+ * Not for use under graph capture. */
+int msig_kernel_timing_arm(int32_t max_launches);
+int msig_kernel_timing_read(double* total_ms, int32_t* launches);
 
 #ifdef __cplusplus
 }

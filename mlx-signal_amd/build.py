@@ -44,9 +44,14 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm toolchain required to build libmsig.so)")
 
 
+# Per-file extras: the matrix-core kernel keeps its MFMA accumulators in
+# ordinary VGPRs (no accumulator-register copies in the loop).
+EXTRA_FLAGS = {"msig_mfma.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [hipcc(), "-c", *CXXFLAGS, str(src), "-o", str(obj)]
+    cmd = [hipcc(), "-c", *CXXFLAGS, *EXTRA_FLAGS.get(src.name, []), str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -74,6 +74,49 @@ static int twiddles(int N, const cf** out) {
   return 0;
 }
 
+// ---------------------------------------------------------------- kernel timing
+// Event pairs handed out to the next calls on the armed device, one pair per
+// fused transform kernel.  The vector is sized at arm time and never grows
+// while armed, so a handed-out pair stays valid until read.
+struct TimingEvents {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  size_t used = 0;
+  int device = -1;
+};
+static std::mutex g_kt_mu;
+static TimingEvents g_kt;
+
+static void timing_release() {
+  for (auto& p : g_kt.ev) {
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  g_kt.ev.clear();
+  g_kt.used = 0;
+  g_kt.device = -1;
+}
+
+static std::pair<hipEvent_t, hipEvent_t>* timing_slot() {
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  if (g_kt.used >= g_kt.ev.size()) return nullptr;
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != g_kt.device) return nullptr;
+  return &g_kt.ev[g_kt.used++];
+}
+
+// Launch `body` between the two events of a timing slot, if one is armed.
+template <class F>
+static hipError_t timed_launch(hipStream_t s, F&& body) {
+  auto* slot = timing_slot();
+  if (slot) {
+    hipError_t e = hipEventRecord(slot->first, s);
+    if (e != hipSuccess) return e;
+  }
+  hipError_t e = body();
+  if (e != hipSuccess || !slot) return e;
+  return hipEventRecord(slot->second, s);
+}
+
 // ---------------------------------------------------------------- finalize
 struct FinArgs {
   const float* ws;
@@ -252,6 +295,8 @@ static bool is_pair(int kind) {
          kind == MSIG_CSD_FRAMES || kind == MSIG_COH_FRAMES;
 }
 
+static int check_precision(const msig_job* j);
+
 static int check_job(const msig_job* j) {
   if (!j) return fail(MSIG_EINVAL, "job is NULL");
   if (j->kind < MSIG_PSD_MEAN || j->kind > MSIG_COH_FRAMES)
@@ -273,7 +318,7 @@ static int check_job(const msig_job* j) {
   if (!pow2(j->nfft) || j->nfft < MSIG_MIN_NFFT || j->nfft > MSIG_MAX_NFFT)
     return fail(MSIG_ENOTSUP, "nfft %d outside the fused power-of-two range [%d, %d]", j->nfft,
                 MSIG_MIN_NFFT, MSIG_MAX_NFFT);
-  return 0;
+  return check_precision(j);
 }
 
 static int nbins_of(const msig_job* j) {
@@ -281,10 +326,37 @@ static int nbins_of(const msig_job* j) {
   return (!cplx && (j->flags & MSIG_F_ONESIDED)) ? j->nfft / 2 + 1 : j->nfft;
 }
 
+// Precision selected by the MSIG_F_MFMA_* flags.
+static int precision_of(const msig_job* j) {
+  if (j->flags & MSIG_F_MFMA_F16) return PREC_F16;
+  if (j->flags & MSIG_F_MFMA_BF16) return PREC_BF16;
+  return PREC_FP32;
+}
+
+// The split-precision form covers the welch mean at nperseg = nfft = 256.
+static int check_precision(const msig_job* j) {
+  const int prec = precision_of(j);
+  if (prec == PREC_FP32) return 0;
+  if ((j->flags & MSIG_F_MFMA_F16) && (j->flags & MSIG_F_MFMA_BF16))
+    return fail(MSIG_EINVAL, "MSIG_F_MFMA_F16 and MSIG_F_MFMA_BF16 are exclusive");
+  if (j->kind != MSIG_PSD_MEAN || (j->flags & MSIG_F_COMPLEX_INPUT) ||
+      j->nfft != MFMA_NFFT || j->nperseg != MFMA_NFFT)
+    return fail(MSIG_ENOTSUP,
+                "the split-precision matrix-core form covers real-input PSD means with "
+                "nperseg = nfft = %d only",
+                MFMA_NFFT);
+  return 0;
+}
+
+static Geo mean_geometry(const msig_job* j) {
+  if (precision_of(j) != PREC_FP32) return geometry_mfma(j->rows, j->nseg, MEAN_MAX_RUN);
+  const bool cplx = j->flags & MSIG_F_COMPLEX_INPUT;
+  return geometry(j->rows, j->nseg, j->nfft, cplx, MEAN_MAX_RUN);
+}
+
 // Mean-kind workspace: the block partials [rows][bpr][ncomp][nfft] float32.
 static size_t mean_ws_bytes(const msig_job* j) {
-  const bool cplx = j->flags & MSIG_F_COMPLEX_INPUT;
-  const Geo g = geometry(j->rows, j->nseg, j->nfft, cplx, MEAN_MAX_RUN);
+  const Geo g = mean_geometry(j);
   return (size_t)j->rows * g.bpr * ncomp(j->kind) * j->nfft * sizeof(float);
 }
 
@@ -397,7 +469,8 @@ int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const 
     return fail(MSIG_EINVAL, "workspace of %zu bytes needed (got %zu)", need, ws_bytes);
   const bool cplx = job->flags & MSIG_F_COMPLEX_INPUT;
   const int N = job->nfft;
-  const Geo g = geometry(job->rows, job->nseg, N, cplx, MEAN_MAX_RUN);
+  const int prec = precision_of(job);
+  const Geo g = mean_geometry(job);
   const cf* tw = nullptr;
   rc = twiddles(N, &tw);
   if (rc) return rc;
@@ -405,7 +478,10 @@ int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const 
   a.ws = static_cast<float*>(workspace);
   const int kk = job->kind == MSIG_PSD_MEAN ? K_PSD : (job->kind == MSIG_CSD_MEAN ? K_CSD : K_COH);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipError_t e = dispatch_mean(N, kk, cplx, a, job->rows, s);
+  hipError_t e = timed_launch(s, [&] {
+    return prec != PREC_FP32 ? launch_mean_mfma256(prec, a, job->rows, s)
+                             : dispatch_mean(N, kk, cplx, a, job->rows, s);
+  });
   if (e != hipSuccess) return hip_fail(e, "fused mean kernel launch");
   FinArgs f;
   f.ws = a.ws;
@@ -460,7 +536,8 @@ int msig_spectral_frames(const msig_job* job, const void* x, const void* y, cons
     default: ok = F_COH; break;
   }
   a.oscale = ok == F_COMPLEX ? (float)std::sqrt(job->scale) : (float)job->scale;
-  hipError_t e = dispatch_frames(N, ok, cplx, a, job->rows, static_cast<hipStream_t>(stream));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = timed_launch(s, [&] { return dispatch_frames(N, ok, cplx, a, job->rows, s); });
   if (e != hipSuccess) return hip_fail(e, "fused frames kernel launch");
   return 0;
 }
@@ -555,6 +632,8 @@ int msig_batch_create(const msig_batch_job* jobs, int32_t njobs, const msig_tuni
     int rc = check_job(j);
     if (rc) return fail(rc, "job %d: %s", i, msig_last_error());
     if (!is_mean(j->kind)) return fail(MSIG_EINVAL, "job %d: kind %d is not a mean kind", i, j->kind);
+    if (precision_of(j) != PREC_FP32)
+      return fail(MSIG_ENOTSUP, "job %d: batch plans run the FP32 form only", i);
     const bool pair = is_pair(j->kind);
     if (!bj.x || !bj.win || !bj.out0 || (pair && !bj.y))
       return fail(MSIG_EINVAL, "job %d: NULL input/output pointer", i);
@@ -706,6 +785,51 @@ int msig_mean_workspace_bytes(int32_t kind, int64_t rows, int64_t n, int32_t npe
     return fail(MSIG_EINVAL, "kind %d has no mean workspace", kind);
   msig_job j = op_job(kind, rows, n, nperseg, hop, 0);
   return msig_workspace_bytes(&j, bytes);
+}
+
+int msig_kernel_timing_arm(int32_t max_launches) {
+  if (max_launches < 0 || max_launches > (1 << 20))
+    return fail(MSIG_EINVAL, "max_launches %d outside [0, 2^20]", max_launches);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  timing_release();
+  for (int32_t i = 0; i < max_launches; ++i) {
+    hipEvent_t a = nullptr, b = nullptr;
+    e = hipEventCreate(&a);
+    if (e == hipSuccess) {
+      e = hipEventCreate(&b);
+      if (e != hipSuccess) (void)hipEventDestroy(a);
+    }
+    if (e != hipSuccess) {
+      timing_release();
+      return hip_fail(e, "hipEventCreate");
+    }
+    g_kt.ev.emplace_back(a, b);
+  }
+  g_kt.device = dev;
+  return 0;
+}
+
+int msig_kernel_timing_read(double* total_ms, int32_t* launches) {
+  if (!total_ms || !launches) return fail(MSIG_EINVAL, "NULL output pointer");
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  double tot = 0.0;
+  for (size_t i = 0; i < g_kt.used; ++i) {
+    hipError_t e = hipEventSynchronize(g_kt.ev[i].second);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, g_kt.ev[i].first, g_kt.ev[i].second);
+    if (e != hipSuccess) {
+      timing_release();
+      return hip_fail(e, "kernel timing events");
+    }
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = (int32_t)g_kt.used;
+  timing_release();
+  return 0;
 }
 
 }  // extern "C"

@@ -40,6 +40,27 @@ inline Geo geometry(int64_t rows, int64_t nseg, int N, bool cplx, int maxrun,
 constexpr int MEAN_MAX_RUN = 32;    // <= 64 segments summed sequentially in FP32
 constexpr int FRAMES_MAX_RUN = 8;
 
+// Split-precision matrix-core form (nfft = nperseg = 256, welch mean): one
+// wave per segment pair, so a 256-thread block carries 4 groups.
+enum : int { PREC_FP32 = 0, PREC_F16 = 1, PREC_BF16 = 2 };
+constexpr int MFMA_NFFT = 256;
+
+inline Geo geometry_mfma(int64_t rows, int64_t nseg, int maxrun, int64_t target_blocks = 2048) {
+  Geo g;
+  g.gpb = BT / 64;
+  g.units = (int32_t)((nseg + 1) / 2);
+  int64_t bprt = (target_blocks + rows - 1) / rows;
+  if (bprt < 1) bprt = 1;
+  int64_t run = (g.units + g.gpb * bprt - 1) / (g.gpb * bprt);
+  if (run < 1) run = 1;
+  if (run > maxrun) run = maxrun;
+  g.run = (int32_t)run;
+  g.bpr = (int32_t)((g.units + (int64_t)g.gpb * run - 1) / ((int64_t)g.gpb * run));
+  return g;
+}
+
+hipError_t launch_mean_mfma256(int prec, const KArgs& a, int64_t rows, hipStream_t s);
+
 // One pair of launchers per fused FFT size, defined in msig_n<N>.hip.
 #define MSIG_DECLARE_LAUNCHERS(NV)                                                         \
   hipError_t launch_mean_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows,  \

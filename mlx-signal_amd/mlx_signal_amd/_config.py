@@ -11,6 +11,14 @@ Dispatch modes
   ``NotImplementedError`` instead of falling back.  ``"mlx"`` is accepted as
   an alias so code written for the reference keeps working.
 - ``"scipy"``: always use scipy.signal (results still come back as tensors).
+
+Precision (no counterpart in the reference)
+-------------------------------------------
+- ``"fp32"`` (default): the FP32 kernels.
+- ``"fp16_split"`` / ``"bf16_split"``: welch means with
+  ``nperseg == nfft == 256`` (real input, constant or no detrend) run their
+  two 16-point DFT stages on the matrix cores with hi + lo split FP16 / BF16
+  operands and FP32 accumulation.  Every other call keeps the FP32 kernels.
 """
 
 from __future__ import annotations
@@ -30,6 +38,7 @@ __all__ = [
 
 _DISPATCH_MODES = ("auto", "gpu", "mlx", "scipy")
 _FLOAT64_MODES = ("downcast", "strict")
+_PRECISIONS = ("fp32", "fp16_split", "bf16_split")
 
 
 class FallbackWarning(UserWarning):
@@ -48,6 +57,7 @@ class Config:
     float64: str = "downcast"  # or "strict" (raise on float64 input)
     warn_on_downcast: bool = True
     warn_on_fallback: bool = True
+    precision: str = "fp32"  # "fp16_split" / "bf16_split": matrix-core 256-point welch
 
 
 _config = Config()
@@ -67,6 +77,8 @@ def set_config(**kwargs) -> None:
         raise ValueError(f"dispatch must be one of {_DISPATCH_MODES}")
     if kwargs.get("float64", _config.float64) not in _FLOAT64_MODES:
         raise ValueError(f"float64 must be one of {_FLOAT64_MODES}")
+    if kwargs.get("precision", _config.precision) not in _PRECISIONS:
+        raise ValueError(f"precision must be one of {_PRECISIONS}")
     size = kwargs.get("gpu_min_size", _config.gpu_min_size)
     if not isinstance(size, int) or isinstance(size, bool) or size < 0:
         raise ValueError("gpu_min_size must be a non-negative integer")

@@ -35,6 +35,9 @@ F_COMPLEX_INPUT = 0x1
 F_ONESIDED = 0x2
 F_APPLY_FACTOR = 0x4
 F_DETREND_CONST = 0x8
+F_MFMA_F16 = 0x10
+F_MFMA_BF16 = 0x20
+MFMA_NFFT = 256
 
 
 class MsigJob(ctypes.Structure):
@@ -106,6 +109,8 @@ SIGNATURES = {
     "msig_batch_run": (ctypes.c_int, [_vp, _vp, _vp, _sz, _vp]),
     "msig_batch_launches": (ctypes.c_int, [_vp]),
     "msig_batch_destroy": (ctypes.c_int, [_vp]),
+    "msig_kernel_timing_arm": (ctypes.c_int, [_i32]),
+    "msig_kernel_timing_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i32)]),
 }
 
 _lib = None

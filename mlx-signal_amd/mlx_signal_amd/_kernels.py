@@ -104,20 +104,38 @@ def _check_inputs(x2, y2, win, nperseg):
         raise ValueError("window must live on the input's device")
 
 
+_PRECISION_FLAGS = {"fp16_split": _hip.F_MFMA_F16, "bf16_split": _hip.F_MFMA_BF16}
+
+
+def mfma_eligible(kind: str, x2, nperseg: int, nfft: int) -> bool:
+    """Shapes the split-precision matrix-core form covers: real-input welch
+    means with nperseg == nfft == 256."""
+    return (kind == "psd" and x2.dtype == torch.float32 and nperseg == _hip.MFMA_NFFT
+            and nfft == _hip.MFMA_NFFT)
+
+
 def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, onesided=True,
-                  apply_factor=False, scale=1.0):
+                  apply_factor=False, scale=1.0, precision="fp32"):
     """Mean over segments of |X|^2 ("psd"), conj(X) Y ("csd"), the
     (Pxx, Pyy, Pxy) trio ("coh_trio") or |Pxy|^2/(Pxx Pyy) ("coherence"),
     times ``scale`` (and the onesided factor when ``apply_factor``).
-    x2 / y2: (rows, n).  Returns a tuple of (rows, bins) tensors."""
+    x2 / y2: (rows, n).  Returns a tuple of (rows, bins) tensors.
+
+    ``precision`` "fp16_split" / "bf16_split" runs the matrix-core form where
+    :func:`mfma_eligible` holds and the FP32 kernels elsewhere."""
     x2 = _rows(x2)
     y2 = None if y2 is None else _rows(y2)
     _check_inputs(x2, y2, win, nperseg)
     code = _MEAN_KINDS[kind]
     if code != _hip.PSD_MEAN and y2 is None:
         raise ValueError(f"{kind} needs two signals")
+    if precision != "fp32" and precision not in _PRECISION_FLAGS:
+        raise ValueError(f"unknown precision {precision!r}")
     lib = _hip.load()
     job = _job(code, x2, y2, nperseg, nstep, nfft, detrend, onesided, apply_factor, scale)
+    mfma = precision != "fp32" and mfma_eligible(kind, x2, nperseg, nfft)
+    if mfma:
+        job.flags |= _PRECISION_FLAGS[precision]
     cplx = x2.dtype == torch.complex64
     bins = nfft // 2 + 1 if (onesided and not cplx) else nfft
     rows = x2.shape[0]
@@ -139,7 +157,7 @@ def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, oneside
                                 _hip.ptr(o[0]), _hip.ptr(o[1]), _hip.ptr(o[2]), _hip.ptr(ws),
                                 ws.numel(), _hip.stream_of(x2))
     _hip.check(rc, f"spectral_mean({kind})")
-    launches["mean_" + kind] += 1
+    launches["mean_" + kind + ("_" + precision if mfma else "")] += 1
     return outs
 
 
@@ -307,3 +325,18 @@ def coherence_mean(x, y, win, nperseg: int, hop: int, detrend):
     _hip.check(rc, "coherence_mean")
     launches["op_coherence_mean"] += 1
     return pxx.reshape(*batch, m1), pyy.reshape(*batch, m1), pxy.reshape(*batch, m1)
+
+
+def kernel_timing_arm(max_launches: int) -> None:
+    """Record HIP events around the fused transform kernel of the next
+    ``max_launches`` mean / frames calls on the current device (instrumentation)."""
+    _hip.check(_hip.load().msig_kernel_timing_arm(int(max_launches)), "kernel timing")
+
+
+def kernel_timing_read() -> tuple[float, int]:
+    """(summed kernel milliseconds, launches timed) since the last arm; disarms."""
+    tot = ctypes.c_double(0.0)
+    n = ctypes.c_int32(0)
+    _hip.check(_hip.load().msig_kernel_timing_read(ctypes.byref(tot), ctypes.byref(n)),
+               "kernel timing")
+    return tot.value, n.value

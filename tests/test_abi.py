@@ -60,3 +60,16 @@ def test_bad_descriptor_rejected_without_gpu():
     job.nperseg = 100
     job.nseg = (1000 - 100) // 128 + 1
     assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == _hip.MSIG_ENOTSUP
+
+
+def test_kernel_timing_contract_without_gpu():
+    if not _hip.LIB_PATH.exists():
+        pytest.skip("libmsig.so not built")
+    lib = _hip.load()
+    assert lib.msig_kernel_timing_arm(-1) == _hip.MSIG_EINVAL
+    tot = ctypes.c_double(-1.0)
+    n = ctypes.c_int32(-1)
+    # nothing armed: reads zero launches without touching a device
+    assert lib.msig_kernel_timing_read(ctypes.byref(tot), ctypes.byref(n)) == 0
+    assert (tot.value, n.value) == (0.0, 0)
+    assert lib.msig_kernel_timing_read(None, ctypes.byref(n)) == _hip.MSIG_EINVAL

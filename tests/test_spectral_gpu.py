@@ -335,3 +335,17 @@ def test_torch_device_input_passthrough(rng):
     f, p = msig.welch(x, nperseg=1024)
     assert p.device == x.device
     assert_peak_close(p, oracle.welch(x.cpu().numpy(), nperseg=1024)[1], TOL)
+
+
+def test_kernel_timing_events(rng):
+    from mlx_signal_amd import _kernels
+
+    x = torch.from_numpy(make_sig(rng, (4, 1 << 16))).cuda()
+    ref = msig.welch(x, nperseg=1024)[1]
+    _kernels.kernel_timing_arm(2)
+    for _ in range(3):  # the third call finds no free event pair and is not timed
+        p = msig.welch(x, nperseg=1024)[1]
+    tot, n = _kernels.kernel_timing_read()
+    assert n == 2 and tot > 0.0
+    assert torch.equal(p, ref)
+    assert _kernels.kernel_timing_read() == (0.0, 0)
