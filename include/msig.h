@@ -212,6 +212,26 @@ int msig_batch_destroy(msig_batch* plan);
 int msig_mean_workspace_bytes(int32_t kind, int64_t rows, int64_t n, int32_t nperseg, int32_t hop,
                               size_t* bytes);
 
+/* istft, inverse half (reference: src/mlx_signal_processing/_stft_metal.py:753-791, the
+ * inverse kernel of istft_frames :817-870): z complex64 [rows][nseg][nperseg/2+1]
+ * one-sided spectra; weff float32[nperseg] = window * scale / nperseg;
+ * frames float32 [rows][nseg][nperseg] = irfft(z) * weff per frame (the
+ * imaginary parts of the DC and Nyquist bins are ignored, as irfft does).
+ * nperseg a power of two in [MSIG_MIN_NFFT, MSIG_MAX_NFFT], else MSIG_ENOTSUP. */
+int msig_istft_frames(const void* z, int64_t rows, int32_t nseg, int32_t nperseg,
+                      const float* weff, float* frames, void* stream);
+
+/* Overlap-add with the window-power divide (reference: _stft_metal.py:793-815,
+ * the gather kernel of istft_frames): frames float32 [rows][nseg][nperseg][channels]
+ * (channels 1 = real, 2 = interleaved complex); for i = first + j, j < count:
+ *   out[row][j][c] = sum_s frames[row][s][i - s*hop][c] / norm(i),
+ *   norm(i) = sum_s win[i - s*hop]^2 (the divide only where norm > 1e-10),
+ * summed over the segments covering i in increasing s.  rows <= 65535;
+ * first + count <= nperseg + (nseg - 1) * hop. */
+int msig_overlap_add(const float* frames, int64_t rows, int32_t nseg, int32_t nperseg,
+                     int32_t hop, int32_t channels, const float* win, int64_t first,
+                     int64_t count, float* out, void* stream);
+
 /* Kernel timing (instrumentation; no counterpart in the reference, which
  * times whole calls).  After msig_kernel_timing_arm(k) on a device, the next
  * k calls of msig_spectral_mean / msig_spectral_frames (and the operator-level

@@ -230,6 +230,43 @@ __global__ __launch_bounds__(FIN_BINS * FIN_LANES) void k_finalize_batch(
   }
 }
 
+// ---------------------------------------------------------------- overlap-add
+// out[row][j][c] = sum_s frames[row][s][i - s hop][c] / norm(i), i = first + j,
+// over the segments s that cover sample i, in increasing s; norm(i) =
+// sum_s win[i - s hop]^2, applied only where it exceeds 1e-10 (scipy's rule).
+// A gather: each output sample is written once, so the sum order is fixed.
+// Counterpart of the overlap-add kernel of istft_frames (_stft_metal.py:793-815).
+struct OlaArgs {
+  const float* frames;
+  const float* win;
+  float* out;
+  int64_t first, count;
+  int32_t nseg, nperseg, hop, channels;
+};
+
+__global__ __launch_bounds__(256) void k_overlap_add(OlaArgs o) {
+  const int64_t row = blockIdx.y;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // j * channels + c
+  const int64_t total = o.count * o.channels;
+  if (idx >= total) return;
+  const int c = (int)(idx % o.channels);
+  const int64_t i = o.first + idx / o.channels;
+  int64_t s_hi = i / o.hop;
+  if (s_hi > o.nseg - 1) s_hi = o.nseg - 1;
+  const int64_t s_lo = i >= o.nperseg ? (i - o.nperseg) / o.hop + 1 : 0;
+  const MSIG_GLOBAL float* fr =
+      as_global(o.frames) + row * (int64_t)o.nseg * o.nperseg * o.channels;
+  const MSIG_GLOBAL float* w = as_global(o.win);
+  float acc = 0.f, nrm = 0.f;
+  for (int64_t s = s_lo; s <= s_hi; ++s) {
+    const int64_t off = i - s * o.hop;
+    acc += fr[(s * o.nperseg + off) * o.channels + c];
+    const float wv = w[off];
+    nrm = fmaf(wv, wv, nrm);
+  }
+  as_global_mut(o.out)[row * total + idx] = acc / (nrm > 1e-10f ? nrm : 1.f);
+}
+
 // ---------------------------------------------------------------- dispatch
 static bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
@@ -258,6 +295,17 @@ static hipError_t dispatch_frames(int N, int ok, bool cplx, const KArgs& a, int6
 #define C(NV) \
   case NV:    \
     return launch_frames_##NV(ok, cplx, half, a, rows, s);
+    C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+  }
+  return hipErrorInvalidValue;
+}
+
+static hipError_t dispatch_istft(int N, const KArgs& a, int64_t rows, hipStream_t s) {
+  switch (N) {
+#define C(NV) \
+  case NV:    \
+    return launch_istft_##NV(a, rows, s);
     C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
   }
@@ -785,6 +833,58 @@ int msig_mean_workspace_bytes(int32_t kind, int64_t rows, int64_t n, int32_t npe
     return fail(MSIG_EINVAL, "kind %d has no mean workspace", kind);
   msig_job j = op_job(kind, rows, n, nperseg, hop, 0);
   return msig_workspace_bytes(&j, bytes);
+}
+
+int msig_istft_frames(const void* z, int64_t rows, int32_t nseg, int32_t nperseg,
+                      const float* weff, float* frames, void* stream) {
+  if (!z || !weff || !frames) return fail(MSIG_EINVAL, "NULL pointer");
+  if (rows < 1 || nseg < 1) return fail(MSIG_EINVAL, "rows and nseg must be >= 1");
+  if (!pow2(nperseg) || nperseg < MSIG_MIN_NFFT || nperseg > MSIG_MAX_NFFT)
+    return fail(MSIG_ENOTSUP, "nperseg %d outside the fused power-of-two range [%d, %d]",
+                nperseg, MSIG_MIN_NFFT, MSIG_MAX_NFFT);
+  const int N = nperseg;
+  const cf* tw = nullptr;
+  int rc = twiddles(N, &tw);
+  if (rc) return rc;
+  const Geo g = geometry(rows, nseg, N, false, FRAMES_MAX_RUN);
+  KArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.x = z;
+  a.win = weff;
+  a.twt = tw;
+  a.out0 = frames;
+  a.nseg = nseg;
+  a.nperseg = N;
+  a.hop = N;
+  a.units = g.units;
+  a.run = g.run;
+  a.bpr = g.bpr;
+  a.nbins = N / 2 + 1;
+  hipError_t e = dispatch_istft(N, a, rows, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "inverse frames kernel launch");
+  return 0;
+}
+
+int msig_overlap_add(const float* frames, int64_t rows, int32_t nseg, int32_t nperseg,
+                     int32_t hop, int32_t channels, const float* win, int64_t first,
+                     int64_t count, float* out, void* stream) {
+  if (!frames || !win || !out) return fail(MSIG_EINVAL, "NULL pointer");
+  if (rows < 1 || nseg < 1 || nperseg < 1 || hop < 1)
+    return fail(MSIG_EINVAL, "rows, nseg, nperseg and hop must be >= 1");
+  if (channels != 1 && channels != 2) return fail(MSIG_EINVAL, "channels must be 1 or 2");
+  const int64_t outlen = nperseg + (int64_t)(nseg - 1) * hop;
+  if (first < 0 || count < 0 || first + count > outlen)
+    return fail(MSIG_EINVAL, "samples [%lld, %lld) outside the %lld-sample output",
+                (long long)first, (long long)(first + count), (long long)outlen);
+  if (rows > 65535) return fail(MSIG_EINVAL, "rows > 65535");
+  if (count == 0) return 0;
+  OlaArgs o{frames, win, out, first, count, nseg, nperseg, hop, channels};
+  const int64_t total = count * channels;
+  const dim3 grid((unsigned)((total + 255) / 256), (unsigned)rows), block(256);
+  hipLaunchKernelGGL(k_overlap_add, grid, block, 0, static_cast<hipStream_t>(stream), o);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "overlap-add kernel launch");
+  return 0;
 }
 
 int msig_kernel_timing_arm(int32_t max_launches) {

@@ -49,15 +49,45 @@ __device__ __forceinline__ void group_sync() {
   }
 }
 
-// Sum a pair of floats over the TPF threads of a group.
+// Lane exchanges within a wave without LDS addressing.  dpp_f<CTRL> reads
+// the lane given by a DPP control (quad permutes, half-row / row mirrors);
+// swz16_f reads lane ^ 16 within each 32; lane_f reads one lane.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+
+__device__ __forceinline__ float swz16_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x401F));
+}
+
+__device__ __forceinline__ float lane_f(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
+// Sum over aligned groups of W lanes (W a power of two <= 64); every lane of
+// a group ends with the group's sum.  Each step adds the value of a partner
+// lane: xor 1, xor 2 (quad permutes), the half-row mirror (i <-> 7 - i) and
+// row mirror (i <-> 15 - i), which after the earlier steps pair up
+// equal-sized partial sums, xor 16, and the two 32-lane halves.
+template <int W>
+__device__ __forceinline__ float lanes_sum(float v) {
+  if constexpr (W >= 2) v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+  if constexpr (W >= 4) v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+  if constexpr (W >= 8) v += dpp_f<0x141>(v);  // row_half_mirror
+  if constexpr (W >= 16) v += dpp_f<0x140>(v); // row_mirror
+  if constexpr (W >= 32) v += swz16_f(v);
+  if constexpr (W >= 64) v = lane_f(v, 0) + lane_f(v, 32);
+  return v;
+}
+
+// Sum two values over the TPF threads of an FFT group.
 template <int TPF>
 __device__ __forceinline__ void group_sum2(float& a, float& b, float* red) {
   constexpr int W = TPF < 64 ? TPF : 64;
-#pragma unroll
-  for (int o = W / 2; o >= 1; o >>= 1) {
-    a += __shfl_xor(a, o, W);
-    b += __shfl_xor(b, o, W);
-  }
+  a = lanes_sum<W>(a);
+  b = lanes_sum<W>(b);
   if constexpr (TPF > 64) {
     constexpr int NW = TPF / 64;  // waves per group
     const int tid = threadIdx.x;
@@ -194,6 +224,78 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[c][e] = 0.f;
 
+  // one unit: z already taken, detrend + window + FFT + accumulate
+  auto unit_body = [&](cf (&zx)[E], cf (&zy)[E], bool active) {
+    if constexpr (KIND == K_PSD) {
+      prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
+      fft_passes<N, E, TH::TS>(t, zx, tw, lds);
+      if (active) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) acc[0][e] = fmaf(zx[e].im, zx[e].im, fmaf(zx[e].re, zx[e].re, acc[0][e]));
+      }
+    } else {
+      prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
+      fft_passes<N, E, TH::TS>(t, zx, tw, lds);
+      group_sync<TPF>();  // last-pass reads of zx done before zy's stores
+      prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
+      fft_passes<N, E, TH::TS>(t, zy, tw, lds);
+      if (active) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          // conj(zx) zy accumulated with four FMAs per bin
+          float qr = fmaf(zx[e].re, zy[e].re, acc[K_CSD == KIND ? 0 : 2][e]);
+          float qi = fmaf(zx[e].re, zy[e].im, acc[K_CSD == KIND ? 1 : 3][e]);
+          qr = fmaf(zx[e].im, zy[e].im, qr);
+          qi = fmaf(-zx[e].im, zy[e].re, qi);
+          if constexpr (KIND == K_CSD) {
+            acc[0][e] = qr;
+            acc[1][e] = qi;
+          } else {
+            acc[0][e] = fmaf(zx[e].im, zx[e].im, fmaf(zx[e].re, zx[e].re, acc[0][e]));
+            acc[1][e] = fmaf(zy[e].im, zy[e].im, fmaf(zy[e].re, zy[e].re, acc[1][e]));
+            acc[2][e] = qr;
+            acc[3][e] = qi;
+          }
+        }
+      }
+    }
+    group_sync<TPF>();  // reads of this unit's last pass done before next stores
+  };
+
+  if constexpr (HALF) {
+    // Units [unit0, u1) of this group; those below nseg / 2 have both
+    // segments and run with unconditional loads (the prefetch past the last
+    // of them re-reads that unit, in bounds and unused).  The partner-less
+    // unit of an odd segment count runs once after them.
+    const int u1 = unit0 + a.run < a.units ? unit0 + a.run : a.units;
+    const int full = (int)(a.nseg / 2);
+    const int uf = u1 < full ? u1 : full;
+    if (unit0 < uf) {
+      sx.fetch_full(unit0, true);
+      if constexpr (KIND != K_PSD) sy.fetch_full(unit0, true);
+      for (int unit = unit0; unit < uf; ++unit) {
+        const int nxt = unit + 1 < uf ? unit + 1 : uf - 1;
+        cf zx[E], zy[E];
+        sx.take_full(zx);
+        sx.fetch_full(nxt, false);
+        if constexpr (KIND != K_PSD) {
+          sy.take_full(zy);
+          sy.fetch_full(nxt, false);
+        }
+        unit_body(zx, zy, true);
+      }
+    }
+    if (full >= unit0 && full < u1) {
+      cf zx[E], zy[E];
+      sx.fetch(full, true, a);
+      sx.take(zx);
+      if constexpr (KIND != K_PSD) {
+        sy.fetch(full, true, a);
+        sy.take(zy);
+      }
+      unit_body(zx, zy, true);
+    }
+  } else {
   sx.fetch(unit0, true, a);
   if constexpr (KIND != K_PSD) sy.fetch(unit0, true, a);
   for (int it = 0; it < a.run; ++it) {
@@ -243,6 +345,7 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
       }
     }
     group_sync<TPF>();  // reads of this unit's last pass done before next stores
+  }
   }
 
   // block reduction: groups summed in a fixed order
@@ -456,6 +559,91 @@ __global__ __launch_bounds__(BT, (frames_waves<N, OK>())) void k_frames(const KA
 }
 
 
+// Inverse transform of one-sided spectra, two frames per complex FFT.
+// Frames s and s+1 of a row (X_s, X_{s+1}: nbins_in = N/2 + 1 bins each,
+// Hermitian extension X[k] = conj X[N - k] above N/2, the imaginary parts of
+// the DC and Nyquist bins ignored as irfft does) are packed as
+// Z = X_s + i X_{s+1}; ifft(Z) = conj(fft(conj Z)) / N has real part x_s and
+// imaginary part x_{s+1}.  The frames are written multiplied by weff
+// (= window * scale / N) as [rows][nseg][N] float32, for the overlap-add.
+// Counterpart of the inverse kernel of istft_frames (_stft_metal.py:753-791).
+template <int N, int E>
+__device__ __forceinline__ void istft_body(const KArgs& a, int64_t row, int bir, cf* lds_all,
+                                           const cf* twl) {
+  using PL = Plan<N, E>;
+  constexpr int TPF = PL::TPF;
+  constexpr int GPB = BT / TPF;
+  constexpr int M = N / 2;
+  const int tid = threadIdx.x;
+  const int g = tid / TPF, t = tid % TPF;
+  cf* lds = lds_all + g * PL::LDS;
+  const int unit0 = (bir * GPB + g) * a.run;
+
+  using TH = TwiddleHome<N, E>;
+  cf twr[TH::REGS];
+  const cf* tw;
+  if constexpr (TH::LDS) {
+    tw = twl + t;
+  } else {
+    load_twiddles<N, E>(t, as_global(a.twt), twr);
+    tw = twr;
+  }
+  float w[E];  // weff at the output samples this thread holds
+#pragma unroll
+  for (int e = 0; e < E; ++e) w[e] = as_global(a.win)[PL::bin(t, e)];
+  const int nb = a.nbins;  // bins per input frame (N/2 + 1)
+  const MSIG_GLOBAL cf* zrow = as_global(static_cast<const cf*>(a.x)) + row * a.nseg * nb;
+  MSIG_GLOBAL float* orow = as_global_mut(static_cast<float*>(a.out0)) + row * a.nseg * N;
+
+  // X[k] of one frame, k < N, from its one-sided bins
+  auto bin_of = [&](const MSIG_GLOBAL cf* f, int k) -> cf {
+    if (k == 0) return cf{f[0].re, 0.f};
+    if (k == M) return cf{f[M].re, 0.f};
+    if (k < M) return cf{f[k].re, f[k].im};
+    const cf c = cf{f[N - k].re, f[N - k].im};
+    return cf{c.re, -c.im};
+  };
+
+  for (int it = 0; it < a.run; ++it) {
+    const int unit = unit0 + it;
+    if (unit >= a.units) break;  // group-uniform
+    const int64_t s0 = 2 * (int64_t)unit;
+    const bool two = s0 + 1 < a.nseg;
+    const MSIG_GLOBAL cf* f0 = zrow + s0 * nb;
+    const MSIG_GLOBAL cf* f1 = f0 + nb;
+    cf z[E];  // FFT input position t + TPF e
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + TPF * e;
+      const cf xs = bin_of(f0, k);
+      const cf xt = two ? bin_of(f1, k) : cf{0.f, 0.f};
+      // conj(X_s + i X_t)
+      z[e] = cf{xs.re - xt.im, -(xs.im + xt.re)};
+    }
+    fft_passes<N, E, TH::TS>(t, z, tw, lds);
+    // sample n = bin(t, e): x_s = Re F / N, x_t = -Im F / N (1/N folded in w)
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int n = PL::bin(t, e);
+      orow[s0 * N + n] = z[e].re * w[e];
+      if (two) orow[(s0 + 1) * N + n] = -z[e].im * w[e];
+    }
+    group_sync<TPF>();
+  }
+}
+
+template <int N, int E>
+__global__ __launch_bounds__(BT) void k_istft(const KArgs a) {
+  using PL = Plan<N, E>;
+  __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
+  __shared__ cf twl[TwiddleHome<N, E>::LDS_ELEMS];
+  if constexpr (TwiddleHome<N, E>::LDS) {
+    fill_twiddle_lds<N, E>(twl, a.twt);
+    __syncthreads();
+  }
+  istft_body<N, E>(a, blockIdx.x / a.bpr, blockIdx.x % a.bpr, lds_all, twl);
+}
+
 // One entry of a ragged batch's work queue: block `blk` (row * bpr + bir) of
 // job `job`.  Entries of one FFT-size class are walked by a persistent grid.
 struct WorkItem {
@@ -467,8 +655,8 @@ struct WorkItem {
 // grid strides over the class's work queue; every entry runs exactly the
 // block that a standalone launch of its job would run, so each job's result
 // is the same alone and in the batch.
-// The batch kernels keep the per-job descriptor and queue state live across
-// the job loop; one wave fewer than the standalone kernels avoids spills.
+// The batch kernels run at the standalone kernels' occupancy (measured
+// faster on ragged batches than one wave fewer).
 template <int N, int KIND>
 constexpr int batch_waves() {
   return mean_waves<N, KIND>();

@@ -69,7 +69,8 @@ hipError_t launch_mean_mfma256(int prec, const KArgs& a, int64_t rows, hipStream
                                 hipStream_t s);                                             \
   hipError_t launch_mean_batch_##NV(int kind, bool cplx, bool half, const KArgs* jobs,      \
                                     const WorkItem* queue, int32_t nwork, float* ws,        \
-                                    int grid, hipStream_t s);
+                                    int grid, hipStream_t s);                               \
+  hipError_t launch_istft_##NV(const KArgs& a, int64_t rows, hipStream_t s);
 MSIG_DECLARE_LAUNCHERS(16)
 MSIG_DECLARE_LAUNCHERS(32)
 MSIG_DECLARE_LAUNCHERS(64)
@@ -136,6 +137,11 @@ MSIG_DECLARE_LAUNCHERS(4096)
     } else {                                                                               \
       MSIG_KIND_SWITCH(MSIG_LAUNCH_MEAN_BATCH, NV, K_PSD, K_CSD, K_COH, false, false)      \
     }                                                                                      \
+    return hipGetLastError();                                                              \
+  }                                                                                        \
+  hipError_t launch_istft_##NV(const KArgs& a, int64_t rows, hipStream_t s) {              \
+    const dim3 grid((unsigned)(rows * a.bpr)), block(BT);                                  \
+    hipLaunchKernelGGL((k_istft<NV, (NV < 16 ? NV : 16)>), grid, block, 0, s, a);          \
     return hipGetLastError();                                                              \
   }                                                                                        \
   }

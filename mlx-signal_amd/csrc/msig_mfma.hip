@@ -95,36 +95,9 @@ __device__ __forceinline__ f4 split_mma(const typename S::V& ah, const typename 
   return S::mma(ah, bh, d);
 }
 
-// Butterfly steps over the 64 lanes of a wave (every lane ends with the
-// result): xor 1 and xor 2 within quads, half-row and row mirrors, xor 16
-// within 32 lanes, then the two 32-lane halves through readlane.
-template <int CTRL>
-__device__ __forceinline__ float dpp_f(float v) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
-}
-
-__device__ __forceinline__ float swz16_f(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x401F));
-}
-
-__device__ __forceinline__ float lane_f(float v, int l) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-}
-
 __device__ __forceinline__ void wave_sum2(float& a, float& b) {
-  a += dpp_f<0xB1>(a);
-  b += dpp_f<0xB1>(b);
-  a += dpp_f<0x4E>(a);
-  b += dpp_f<0x4E>(b);
-  a += dpp_f<0x141>(a);
-  b += dpp_f<0x141>(b);
-  a += dpp_f<0x140>(a);
-  b += dpp_f<0x140>(b);
-  a += swz16_f(a);
-  b += swz16_f(b);
-  a = lane_f(a, 0) + lane_f(a, 32);
-  b = lane_f(b, 0) + lane_f(b, 32);
+  a = lanes_sum<64>(a);
+  b = lanes_sum<64>(b);
 }
 
 __device__ __forceinline__ float wave_max(float a) {

@@ -110,6 +110,32 @@ struct UnitStream {
     }
   }
 
+  // HALF: a unit whose two segments both exist (unit < nseg / 2).  No bounds
+  // selects; `first` loads the shared half-block, else it is carried.
+  MSIG_HD void fetch_full(int unit, bool first) {
+    static_assert(HALF, "full-unit fetch is the half-hop stream's");
+    const int64_t s0 = 2 * (int64_t)unit;
+    if (first) {
+      half_load(h0, s0, true);
+    } else {
+#pragma unroll
+      for (int e = 0; e < H; ++e) h0[e] = h2[e];
+    }
+    half_load(h1, s0 + 1, true);
+    half_load(h2, s0 + 2, true);
+    pair_ok = true;
+  }
+
+  // HALF: the points of a unit fetched with fetch_full
+  MSIG_HD void take_full(cf* z) const {
+    static_assert(HALF, "full-unit take is the half-hop stream's");
+#pragma unroll
+    for (int e = 0; e < H; ++e) {
+      z[e] = cf{h0[e], h1[e]};
+      z[e + H] = cf{h1[e], h2[e]};
+    }
+  }
+
   // the fetched unit as complex points z[e] <-> sample t + TPF*e
   MSIG_HD void take(cf* z) const {
     if constexpr (HALF) {

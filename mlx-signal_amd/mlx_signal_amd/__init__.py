@@ -16,7 +16,7 @@ from ._config import (
     get_config,
     set_config,
 )
-from .spectral import coherence, csd, periodogram, spectrogram, stft, welch
+from .spectral import coherence, csd, istft, periodogram, spectrogram, stft, welch
 from .windows import get_window
 
 __all__ = [
@@ -28,6 +28,7 @@ __all__ = [
     "csd",
     "get_config",
     "get_window",
+    "istft",
     "periodogram",
     "set_config",
     "spectrogram",

@@ -340,3 +340,38 @@ def kernel_timing_read() -> tuple[float, int]:
     _hip.check(_hip.load().msig_kernel_timing_read(ctypes.byref(tot), ctypes.byref(n)),
                "kernel timing")
     return tot.value, n.value
+
+
+def istft_frames(z3, weff, nperseg: int):
+    """Inverse transform of one-sided spectra, frame by frame, times ``weff``
+    (reference: the inverse kernel of istft_frames, _stft_metal.py:817-870).
+    z3: (rows, nseg, nperseg/2 + 1) complex64; returns (rows, nseg, nperseg)
+    float32 frames."""
+    if z3.dtype != torch.complex64 or z3.dim() != 3 or z3.shape[-1] != nperseg // 2 + 1:
+        raise ValueError("istft_frames takes (rows, nseg, nperseg/2+1) complex64")
+    z3 = z3.contiguous()
+    rows, nseg, _ = z3.shape
+    frames = torch.empty(rows, nseg, nperseg, dtype=torch.float32, device=z3.device)
+    rc = _hip.load().msig_istft_frames(_hip.ptr(z3), rows, nseg, nperseg, _hip.ptr(weff),
+                                      _hip.ptr(frames), _hip.stream_of(z3))
+    _hip.check(rc, "istft_frames")
+    launches["istft_frames"] += 1
+    return frames
+
+
+def overlap_add(frames, win, nperseg: int, hop: int, first: int, count: int):
+    """Gather overlap-add with the window-power divide (reference: the OLA
+    kernel of istft_frames, _stft_metal.py:793-815).  frames: (rows, nseg,
+    nperseg) float32 or complex64; returns (rows, count) of the same dtype,
+    samples first .. first + count - 1 of the overlap-added signal."""
+    cplx = frames.dtype == torch.complex64
+    fr = torch.view_as_real(frames.contiguous()) if cplx else frames.contiguous()
+    rows, nseg = frames.shape[0], frames.shape[1]
+    ch = 2 if cplx else 1
+    out = torch.empty((rows, count, 2) if cplx else (rows, count), dtype=torch.float32,
+                      device=frames.device)
+    rc = _hip.load().msig_overlap_add(_hip.ptr(fr), rows, nseg, nperseg, hop, ch, _hip.ptr(win),
+                                     first, count, _hip.ptr(out), _hip.stream_of(fr))
+    _hip.check(rc, "overlap_add")
+    launches["overlap_add"] += 1
+    return torch.view_as_complex(out) if cplx else out

@@ -16,7 +16,9 @@ reference line it follows cited in its docstring.  Pinning (tests/test_oracle.py
     called on float64 inputs (the reference's own CPU route, spectral.py:433-450)
     over the reference tests' case lists, plus the committed golden fixtures;
   * the transform itself: an O(N^2) DFT on tiny inputs, a bin-centred tone's
-    closed-form peak, Parseval, coherence(x, x) == 1, csd(x, x) == welch(x).
+    closed-form peak, Parseval, coherence(x, x) == 1, csd(x, x) == welch(x);
+  * istft: scipy.signal.istft on complex128 input over the reference tests'
+    cases, and the stft -> istft round trip returning the signal.
 """
 
 from __future__ import annotations
@@ -30,6 +32,7 @@ __all__ = [
     "csd",
     "dft",
     "get_window",
+    "istft",
     "median_bias",
     "onesided_factor",
     "periodogram",
@@ -444,6 +447,94 @@ def stft(x, fs=1.0, window="hann", nperseg=256, noverlap=None, nfft=None, detren
     return spectral_helper(x, None, fs, window, nperseg, noverlap, nfft, detrend,
                            return_onesided, scaling, axis, mode="stft", boundary=boundary,
                            padded=padded)
+
+
+def _istft_axes(ndim: int, freq_axis: int, time_axis: int):
+    """spectral.py:766-785: normalized axes and the permutation to (..., time, freq)."""
+    if ndim < 2:
+        raise ValueError("Input stft must be at least 2d!")
+    if freq_axis == time_axis:
+        raise ValueError("Must specify differing time and frequency axes!")
+    freq_ax = freq_axis + ndim if freq_axis < 0 else freq_axis
+    time_ax = time_axis + ndim if time_axis < 0 else time_axis
+    if not (0 <= freq_ax < ndim and 0 <= time_ax < ndim):
+        raise IndexError("axis out of range")
+    if freq_ax == time_ax:
+        raise ValueError("Must specify differing time and frequency axes!")
+    outer = [i for i in range(ndim) if i not in (freq_ax, time_ax)]
+    return freq_ax, time_ax, outer + [time_ax, freq_ax]
+
+
+def istft_plan(nf, input_onesided=True, nperseg=None, noverlap=None, nfft=None):
+    """Integer plan of istft (spectral.py:888-912): nperseg, nfft, nstep."""
+    n_default = 2 * (nf - 1) if input_onesided else nf
+    if nperseg is None:
+        nperseg = n_default
+    else:
+        nperseg = int(nperseg)
+        if nperseg < 1:
+            raise ValueError("nperseg must be a positive integer")
+    if nfft is None:
+        nfft = nperseg if (input_onesided and nperseg == n_default + 1) else n_default
+    elif nfft < nperseg:
+        raise ValueError("nfft must be greater than or equal to nperseg.")
+    else:
+        nfft = int(nfft)
+    noverlap = nperseg // 2 if noverlap is None else int(noverlap)
+    if noverlap >= nperseg:
+        raise ValueError("noverlap must be less than nperseg.")
+    return nperseg, nfft, nperseg - noverlap
+
+
+def istft(Zxx, fs=1.0, window="hann", nperseg=None, noverlap=None, nfft=None,
+          input_onesided=True, boundary=True, time_axis=-1, freq_axis=-2, scaling="spectrum"):
+    """spectral.py:824-992 in complex128: inverse FFT of every frame, times
+    scale and the window, overlap-add, divide by the window-power sum where it
+    exceeds 1e-10, boundary trim.  Returns (time, x) with the time vector the
+    length of the output's time axis (the reference's documented deviation
+    from scipy for N-d input)."""
+    Z = np.asarray(Zxx).astype(np.complex128)
+    freq_ax, time_ax, perm = _istft_axes(Z.ndim, int(freq_axis), int(time_axis))
+    Z = np.transpose(Z, perm)
+    nseg, nf = Z.shape[-2], Z.shape[-1]
+    nperseg, nfft, nstep = istft_plan(nf, input_onesided, nperseg, noverlap, nfft)
+    if isinstance(window, (str, tuple)):
+        win = get_window(window, nperseg)
+    else:
+        win = np.asarray(window, dtype=np.float64)
+        if win.ndim != 1:
+            raise ValueError("window must be 1-D")
+        if win.shape[0] != nperseg:
+            raise ValueError(f"window must have length of {nperseg}")
+    if scaling == "spectrum":
+        scale = win.sum()
+    elif scaling == "psd":
+        scale = np.sqrt(fs * np.sum(win ** 2))
+    else:
+        raise ValueError(f"Parameter {scaling=} not in ['spectrum', 'psd']!")
+    inv = np.fft.irfft(Z, n=nfft, axis=-1) if input_onesided else np.fft.ifft(Z, n=nfft, axis=-1)
+    xsubs = inv[..., :nperseg] * scale * win
+    outputlength = nperseg + (nseg - 1) * nstep
+    x = np.zeros(Z.shape[:-2] + (outputlength,), dtype=xsubs.dtype)
+    norm = np.zeros(outputlength)
+    for s in range(nseg):
+        x[..., s * nstep:s * nstep + nperseg] += xsubs[..., s, :]
+        norm[s * nstep:s * nstep + nperseg] += win ** 2
+    if boundary:
+        x = x[..., nperseg // 2:-(nperseg // 2)]
+        norm = norm[nperseg // 2:-(nperseg // 2)]
+    if np.sum(norm > 1e-10) != len(norm):
+        warnings.warn("NOLA condition failed, STFT may not be invertible."
+                      + (" Possibly due to missing boundary" if not boundary else ""),
+                      stacklevel=2)
+    x = x / np.where(norm > 1e-10, norm, 1.0)
+    if input_onesided:
+        x = x.real
+    out_time_axis = time_ax - (1 if freq_ax < time_ax else 0)
+    if x.ndim > 1 and out_time_axis != x.ndim - 1:
+        x = np.moveaxis(x, -1, out_time_axis)
+    n_out = x.shape[out_time_axis if x.ndim > 1 else -1]
+    return np.arange(n_out) / float(fs), x
 
 
 def scipy_route(func: str, *args, **kwargs):

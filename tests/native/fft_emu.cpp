@@ -246,6 +246,118 @@ static int check_stream(std::mt19937_64& rng, int nperseg, int hop, int n, int d
   return ok ? 0 : 1;
 }
 
+// The hop == N/2 group loop of k_mean: each group walks units
+// [unit0, min(unit0 + run, units)); units below nseg / 2 use fetch_full /
+// take_full with the next fetch clamped to the group's last full unit, and
+// the partner-less unit of an odd segment count runs once through the
+// general fetch.  Checks coverage (each unit once), that every fetch_full
+// stays inside the row, and the mean against the FP64 DFT.
+template <int N>
+static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detrend) {
+  constexpr int E = N < 16 ? N : 16;
+  constexpr int TPF = N / E;
+  const int hop = N / 2;
+  const int n = N + hop * (nseg - 1) + hop / 3;  // samples past the last segment are unused
+  std::normal_distribution<double> nd;
+  std::vector<float> x(n + 2 * N, NAN);
+  for (int i = 0; i < n; ++i) x[i] = (float)(nd(rng) + 0.5 * std::sin(0.3 * i) - 1.0);
+  std::vector<cf> table(N);
+  for (int i = 0; i < N; ++i) {
+    const double ang = -2.0 * M_PI * i / N;
+    table[i] = cf{(float)std::cos(ang), (float)std::sin(ang)};
+  }
+  std::vector<double> wd(N);
+  for (int i = 0; i < N; ++i) wd[i] = 0.5 - 0.5 * std::cos(2.0 * M_PI * i / N);
+  msig::KArgs a{};
+  a.nseg = nseg;
+  a.nperseg = N;
+  a.hop = hop;
+  a.units = (int32_t)((nseg + 1) / 2);
+  a.run = run;
+  a.detrend = detrend;
+  const int groups = (a.units + run - 1) / run + 1;  // one group past the end
+  std::vector<int> seen(a.units, 0);
+  std::vector<double> S(N, 0.0);
+  bool in_bounds = true;
+  auto process = [&](std::vector<cf>& z) -> bool {
+    float sr = 0.f, si = 0.f;
+    for (int i = 0; i < N; ++i) {
+      sr += z[i].re;
+      si += z[i].im;
+    }
+    const float mr = detrend ? sr / N : 0.f, mi = detrend ? si / N : 0.f;
+    for (int i = 0; i < N; ++i) z[i] = cf{(z[i].re - mr) * (float)wd[i], (z[i].im - mi) * (float)wd[i]};
+    std::vector<cf> Z;
+    if (!emu_fft<N, E>(table, z, Z)) return false;
+    for (int k = 0; k < N; ++k) S[k] += (double)Z[k].re * Z[k].re + (double)Z[k].im * Z[k].im;
+    return true;
+  };
+  for (int g = 0; g < groups; ++g) {
+    const int unit0 = g * run;
+    const int u1 = unit0 + run < a.units ? unit0 + run : a.units;
+    const int full = (int)(a.nseg / 2);
+    const int uf = u1 < full ? u1 : full;
+    std::vector<msig::UnitStream<N, E, false, true>> st(TPF);
+    for (int t = 0; t < TPF; ++t) {
+      st[t].row = reinterpret_cast<const char*>(x.data());
+      st[t].t = t;
+    }
+    auto full_fetch = [&](int u, bool first) {
+      in_bounds = in_bounds && (int64_t)(2 * u + 3) * hop <= n;
+      for (int t = 0; t < TPF; ++t) st[t].fetch_full(u, first);
+    };
+    if (unit0 < uf) {
+      full_fetch(unit0, true);
+      for (int unit = unit0; unit < uf; ++unit) {
+        const int nxt = unit + 1 < uf ? unit + 1 : uf - 1;
+        std::vector<cf> z(N);
+        for (int t = 0; t < TPF; ++t) {
+          cf zt[E];
+          st[t].take_full(zt);
+          for (int e = 0; e < E; ++e) z[t + TPF * e] = zt[e];
+        }
+        full_fetch(nxt, false);
+        if (!process(z)) return 1;
+        ++seen[unit];
+      }
+    }
+    if (full >= unit0 && full < u1) {
+      std::vector<cf> z(N);
+      for (int t = 0; t < TPF; ++t) {
+        cf zt[E];
+        st[t].fetch(full, true, a);
+        st[t].take(zt);
+        for (int e = 0; e < E; ++e) z[t + TPF * e] = zt[e];
+      }
+      if (!process(z)) return 1;
+      ++seen[full];
+    }
+  }
+  bool once = true;
+  for (int u = 0; u < a.units; ++u) once = once && seen[u] == 1;
+  std::vector<double> P(N, 0.0);
+  for (int s = 0; s < nseg; ++s) {
+    std::vector<cd> seg(N, 0.0);
+    double m = 0;
+    for (int i = 0; i < N; ++i) m += x[(size_t)s * hop + i];
+    m = detrend ? m / N : 0.0;
+    for (int i = 0; i < N; ++i) seg[i] = (x[(size_t)s * hop + i] - m) * wd[i];
+    auto X = dft(seg);
+    for (int k = 0; k < N; ++k) P[k] += std::norm(X[k]);
+  }
+  double peak = 0, err = 0;
+  for (int k = 0; k < N; ++k) {
+    const double got = 0.5 * (S[k] + S[(N - k) & (N - 1)]);
+    peak = std::max(peak, P[k]);
+    err = std::max(err, std::abs(got - P[k]));
+  }
+  const double rel = std::isfinite(err) ? err / peak : 1.0;
+  const bool ok = once && in_bounds && rel < 1e-5;
+  std::printf("half groups N=%5d nseg=%d run=%d detrend=%d once=%d in_bounds=%d rel=%.3e %s\n",
+              N, nseg, run, detrend, (int)once, (int)in_bounds, rel, ok ? "ok" : "FAIL");
+  return ok ? 0 : 1;
+}
+
 int main() {
   std::mt19937_64 rng(1234);
   int bad = 0;
@@ -268,5 +380,13 @@ int main() {
   bad += check_stream<64, false>(rng, 50, 20, 333, 1);
   bad += check_stream<256, false>(rng, 256, 100, 5000, 0);
   bad += check_stream<256, false>(rng, 200, 200, 4000, 1);
+  // the peeled hop == N/2 group loop: odd / even nseg, runs that do and do
+  // not divide the unit count, a run of one
+  bad += check_half_groups<64>(rng, 15, 3, 1);
+  bad += check_half_groups<64>(rng, 16, 3, 0);
+  bad += check_half_groups<256>(rng, 21, 4, 1);
+  bad += check_half_groups<256>(rng, 9, 1, 1);
+  bad += check_half_groups<1024>(rng, 13, 2, 1);
+  bad += check_half_groups<1024>(rng, 1, 4, 1);
   return bad ? 1 : 0;
 }

@@ -197,3 +197,41 @@ def test_composed_detrend_matches_oracle(rng):
         np.testing.assert_allclose(got, ref, rtol=1e-4, atol=1e-5)
     with pytest.raises(ValueError):
         spectral._detrend_and_window(torch.from_numpy(d), "quadratic", torch.from_numpy(w))
+
+
+def test_istft_nola_helper_matches_oracle_rule():
+    """The host NOLA verdict (FP64 window-power sum, cached per geometry)
+    agrees with the oracle's explicit overlap-add of the window power."""
+    from mlx_signal_amd.spectral import _nola_ok
+
+    import oracle
+
+    cases = [("hann", 256, 128, 10, True), ("hann", 256, 256, 10, True),
+             ("hann", 256, 256, 10, False), ("boxcar", 64, 64, 5, False),
+             ("hann", 64, 16, 1, True), ("blackman", 128, 100, 7, False)]
+    for win_name, nps, nstep, nseg, boundary in cases:
+        w = oracle.get_window(win_name, nps)
+        outlen = nps + (nseg - 1) * nstep
+        norm = np.zeros(outlen)
+        for s in range(nseg):
+            norm[s * nstep:s * nstep + nps] += w ** 2
+        if boundary:
+            norm = norm[nps // 2:-(nps // 2)]
+        want = bool(np.sum(norm > 1e-10) == len(norm))
+        assert _nola_ok(w, nps, nstep, nseg, boundary) == want, (win_name, nps, nstep, boundary)
+
+
+def test_istft_scipy_route_on_cpu():
+    """Without a GPU, auto dispatch runs scipy and returns tensors."""
+    import scipy.signal as sps
+
+    import oracle
+
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(2000).astype(np.float32)
+    _, _, z = sps.stft(x, nperseg=128)
+    with msig.config_context(dispatch="scipy"):
+        t, xr = msig.istft(z.astype(np.complex64), nperseg=128)
+    assert isinstance(xr, torch.Tensor) and isinstance(t, torch.Tensor)
+    _, x_ref = oracle.istft(z, nperseg=128)
+    np.testing.assert_allclose(xr.numpy(), x_ref, atol=1e-5 * np.abs(x_ref).max())

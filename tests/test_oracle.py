@@ -7,6 +7,7 @@ invariants.  The planted-mistake tests show that plausible slips (a dropped
 onesided factor, a wrong detrend, an off-by-one segment count) fail here."""
 
 import json
+import warnings
 from pathlib import Path
 
 import numpy as np
@@ -254,3 +255,104 @@ def test_planted_segment_off_by_one_is_caught(monkeypatch):
 def test_planted_wrong_window_is_caught(monkeypatch):
     monkeypatch.setitem(osp._COSINE_SUMS, "hann", (0.5, 0.5, 0.0001))
     assert _golden_welch_err() > 1e-6
+
+
+# ---------------------------------------------------------------------------
+# istft (reference: spectral.py:824-992), pinned to scipy.signal.istft in FP64
+# ---------------------------------------------------------------------------
+
+ISTFT_CASES = [
+    ({}, {}),
+    ({"nperseg": 128}, {"nperseg": 128}),
+    ({"nperseg": 128, "noverlap": 96, "window": "blackman"},
+     {"nperseg": 128, "noverlap": 96, "window": "blackman"}),
+    ({"nperseg": 64, "noverlap": 16}, {"nperseg": 64, "noverlap": 16}),
+    ({"nperseg": 63}, {"nperseg": 63}),                                  # odd nperseg
+    ({"nperseg": 64, "nfft": 128}, {"nperseg": 64, "nfft": 128}),        # padded frames
+    ({"nperseg": 64, "window": ("tukey", 0.25)}, {"nperseg": 64, "window": ("tukey", 0.25)}),
+    ({"nperseg": 64, "boundary": None, "padded": False}, {"nperseg": 64, "boundary": False}),
+    ({"nperseg": 64, "scaling": "psd", "fs": 50.0}, {"nperseg": 64, "scaling": "psd", "fs": 50.0}),
+]
+
+
+@pytest.mark.parametrize("stft_kw,istft_kw", ISTFT_CASES)
+def test_istft_matches_scipy_fp64(stft_kw, istft_kw):
+    import scipy.signal as sps
+
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(1500)
+    _, _, z = sps.stft(x, **stft_kw)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        t_ref, x_ref = sps.istft(z, **istft_kw)
+        t, xo = oracle.istft(z, **istft_kw)
+    np.testing.assert_allclose(xo, x_ref, rtol=0, atol=1e-12 * max(1.0, np.abs(x_ref).max()))
+    np.testing.assert_allclose(t, t_ref, rtol=1e-15)
+
+
+def test_istft_two_sided_complex_matches_scipy():
+    import scipy.signal as sps
+
+    rng = np.random.default_rng(22)
+    x = rng.standard_normal(900) + 1j * rng.standard_normal(900)
+    _, _, z = sps.stft(x, nperseg=64, return_onesided=False)
+    _, x_ref = sps.istft(z, nperseg=64, input_onesided=False)
+    _, xo = oracle.istft(z, nperseg=64, input_onesided=False)
+    assert np.iscomplexobj(xo)
+    np.testing.assert_allclose(xo, x_ref, rtol=0, atol=1e-12)
+
+
+def test_istft_axes_match_scipy():
+    import scipy.signal as sps
+
+    rng = np.random.default_rng(23)
+    x = rng.standard_normal((3, 2, 700))
+    _, _, z = sps.stft(x, nperseg=64)                 # (3, 2, freq, time)
+    zt = np.moveaxis(z, -2, 0)                        # (freq, 3, 2, time)
+    _, x_ref = sps.istft(zt, nperseg=64, freq_axis=0, time_axis=-1)
+    t, xo = oracle.istft(zt, nperseg=64, freq_axis=0, time_axis=-1)
+    np.testing.assert_allclose(xo, x_ref, rtol=0, atol=1e-12)
+    assert len(t) == xo.shape[-1]                     # time vector = output time axis
+
+
+def test_istft_round_trip_recovers_signal():
+    import scipy.signal as sps
+
+    rng = np.random.default_rng(24)
+    x = rng.standard_normal((2, 4096))
+    for nps, nov in ((256, 128), (256, 192), (1024, 512)):
+        _, _, z = sps.stft(x, nperseg=nps, noverlap=nov)
+        _, xr = oracle.istft(z, nperseg=nps, noverlap=nov)
+        np.testing.assert_allclose(xr[..., :4096], x, rtol=0, atol=1e-12)
+
+
+def test_istft_nola_warning_and_errors():
+    z = np.zeros((33, 10), np.complex128)
+    with pytest.warns(UserWarning, match="NOLA"):
+        oracle.istft(z, window=np.zeros(64), nperseg=64)
+    with pytest.raises(ValueError):
+        oracle.istft(np.zeros(10))
+    with pytest.raises(ValueError):
+        oracle.istft(z, freq_axis=0, time_axis=0)
+    with pytest.raises(ValueError):
+        oracle.istft(z, nperseg=64, noverlap=64)
+    with pytest.raises(ValueError):
+        oracle.istft(z, nperseg=64, nfft=32)
+    with pytest.raises(ValueError):
+        oracle.istft(z, scaling="bogus")
+
+
+def test_istft_planted_missing_norm_divide_fails():
+    """Dropping the window-power divide breaks the round trip."""
+    import scipy.signal as sps
+
+    rng = np.random.default_rng(25)
+    x = rng.standard_normal(2048)
+    _, _, z = sps.stft(x, nperseg=128)
+    win = oracle.get_window("hann", 128)
+    xs = np.fft.irfft(z.T, n=128) * win.sum() * win
+    out = np.zeros(128 + 64 * (xs.shape[0] - 1))
+    for s in range(xs.shape[0]):
+        out[s * 64:s * 64 + 128] += xs[s]
+    out = out[64:-64]
+    assert np.abs(out[:2048] - x).max() > 1e-2
