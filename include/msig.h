@@ -232,6 +232,19 @@ int msig_overlap_add(const float* frames, int64_t rows, int32_t nseg, int32_t np
                      int32_t hop, int32_t channels, const float* win, int64_t first,
                      int64_t count, float* out, void* stream);
 
+/* Per-column median over segments (reference: the median average of csd,
+ * src/mlx_signal_processing/spectral.py:469-477): frames float32 [rows][nseg][cols];
+ *   out[row][c] = median_s frames[row][s][c] * scale * f(c),
+ * numpy's median (mean of the two middle values for an even nseg; NaN when
+ * the column holds a NaN), f(c) = 2 when apply_factor and the column's bin
+ * b = c / channels is neither 0 nor nfft/2, else 1 (channels 2: interleaved
+ * complex, real and imaginary parts take their own medians).
+ * nseg <= MSIG_MEDIAN_MAX_NSEG, else MSIG_ENOTSUP. */
+#define MSIG_MEDIAN_MAX_NSEG 16000
+int msig_median_frames(const float* frames, int64_t rows, int32_t nseg, int32_t cols,
+                       int32_t channels, double scale, int32_t apply_factor, int32_t nfft,
+                       float* out, void* stream);
+
 /* Kernel timing (instrumentation; no counterpart in the reference, which
  * times whole calls).  After msig_kernel_timing_arm(k) on a device, the next
  * k calls of msig_spectral_mean / msig_spectral_frames (and the operator-level

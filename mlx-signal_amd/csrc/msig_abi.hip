@@ -267,6 +267,211 @@ __global__ __launch_bounds__(256) void k_overlap_add(OlaArgs o) {
   as_global_mut(o.out)[row * total + idx] = acc / (nrm > 1e-10f ? nrm : 1.f);
 }
 
+// ---------------------------------------------------------------- median
+// Per-column median over segments: frames [rows][nseg][cols] float32 ->
+// out[row][col], numpy's median (the mean of the two middle values for an
+// even count; NaN if the column holds a NaN) times scale and the onesided
+// factor of the column's bin.
+//
+// A block stages CPB columns of one row in LDS as order-preserving uint32
+// keys: one read of the frames, a thread per segment so every LDS write
+// instruction covers consecutive addresses.  Each column's key range
+// [kmin, kmax] is found first; the rank (nseg - 1) / 2 key is then selected
+// by radix passes over the range-relative keys k - kmin, up to 11 bits per
+// pass with LDS-atomic histograms.  Digits cut from the column's own range
+// spread its keys over the buckets (a column of power values spans a few
+// binary exponents only, which fixed top-bit digits would crowd into a few
+// buckets), and a constant column needs no pass.  For an even count one
+// more pass finds the next order statistic.  Counterpart of the median
+// average of csd (spectral.py:469-477, mx.median over the segment axis).
+constexpr int MED_BT = 256;
+constexpr int MED_MAX_CPB = 16;
+constexpr int MED_BITS = 11;
+constexpr int MED_BUCKETS = 1 << MED_BITS;
+// histogram slot of bucket b: one pad word per 32 buckets, so the per-lane
+// sums over contiguous bucket ranges (lane l reads l * per + k) hit
+// distinct banks
+constexpr int MED_HSTRIDE = MED_BUCKETS + MED_BUCKETS / 32;
+__device__ __forceinline__ int med_slot(int b) { return b + (b >> 5); }
+constexpr int MED_LDS_BUDGET = 80 << 10;   // bytes per block: two blocks per CU
+
+struct MedArgs {
+  const float* in;
+  float* out;
+  int64_t rows;
+  int32_t nseg, cols, cpb, ncb, channels, apply_factor, nfft, stride;
+  float scale;
+};
+
+// Column stride >= nseg with stride = 64 / cpb (mod 64): a staging write
+// instruction covers 64 / cpb consecutive segments of each of the cpb
+// columns, which then fall in 64 distinct banks.
+__host__ __device__ __forceinline__ int med_stride(int nseg, int cpb) {
+  const int target = cpb > 1 ? 64 / cpb : 0;
+  return nseg + ((target - nseg % 64) % 64 + 64) % 64;
+}
+
+__host__ __forceinline__ size_t med_lds_bytes(int cpb, int stride) {
+  return ((size_t)cpb * stride + (size_t)cpb * MED_HSTRIDE + MED_BT + 7 * MED_MAX_CPB) * 4;
+}
+
+__device__ __forceinline__ uint32_t med_key(float v) {
+  const uint32_t b = __float_as_uint(v);
+  return b ^ ((b & 0x80000000u) ? 0xFFFFFFFFu : 0x80000000u);
+}
+
+__device__ __forceinline__ float med_val(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k ^ 0x80000000u) : ~k);
+}
+
+__global__ __launch_bounds__(MED_BT) void k_median(MedArgs a) {
+  extern __shared__ uint32_t med_smem[];
+  const int cpb = a.cpb, tpc = MED_BT / cpb, nseg = a.nseg, stride = a.stride;
+  uint32_t* tile = med_smem;                                   // [cpb][stride]
+  uint32_t* hist = tile + cpb * stride;                        // [cpb][MED_HSTRIDE]
+  uint32_t* part = hist + cpb * MED_HSTRIDE;                   // [MED_BT]
+  uint32_t* sel_bucket = part + MED_BT;                        // [MED_MAX_CPB] x 7
+  uint32_t* sel_rank = sel_bucket + MED_MAX_CPB;
+  uint32_t* cnt_le = sel_rank + MED_MAX_CPB;
+  uint32_t* min_gt = cnt_le + MED_MAX_CPB;
+  uint32_t* has_nan = min_gt + MED_MAX_CPB;
+  uint32_t* kmin_s = has_nan + MED_MAX_CPB;
+  uint32_t* kmax_s = kmin_s + MED_MAX_CPB;
+  const int tid = threadIdx.x;
+  const int col = tid / tpc, lt = tid - col * tpc;
+  const int64_t row = blockIdx.x / a.ncb;
+  const int col0 = (int)(blockIdx.x - row * a.ncb) * cpb;
+  const int ncol = a.cols - col0 < cpb ? a.cols - col0 : cpb;  // columns present
+  const MSIG_GLOBAL float* src = as_global(a.in) + row * (int64_t)nseg * a.cols + col0;
+
+  if (tid < MED_MAX_CPB) {
+    has_nan[tid] = 0u;
+    cnt_le[tid] = 0u;
+    min_gt[tid] = 0xFFFFFFFFu;
+    kmin_s[tid] = 0xFFFFFFFFu;
+    kmax_s[tid] = 0u;
+  }
+  __syncthreads();
+  // staging: element i = (segment i / cpb, column i % cpb); a thread's
+  // elements all lie in column tid % cpb (MED_BT is a multiple of cpb).
+  // Eight loads in flight per thread, then their LDS stores.
+  constexpr int UNR = 8;
+  const int total = nseg * cpb;
+  const int my_c = tid % cpb;
+  uint32_t lmin = 0xFFFFFFFFu, lmax = 0u;
+  bool nan_seen = false;
+  for (int base = 0; base < total; base += MED_BT * UNR) {
+    float v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int i = base + u * MED_BT + tid;
+      v[u] = (i < total && my_c < ncol) ? src[(int64_t)(i / cpb) * a.cols + my_c] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int i = base + u * MED_BT + tid;
+      if (i < total) {
+        const uint32_t k = med_key(v[u]);
+        nan_seen |= v[u] != v[u];
+        tile[my_c * stride + i / cpb] = k;
+        lmin = k < lmin ? k : lmin;
+        lmax = k > lmax ? k : lmax;
+      }
+    }
+  }
+  if (lmin != 0xFFFFFFFFu) {
+    atomicMin(&kmin_s[my_c], lmin);
+    atomicMax(&kmax_s[my_c], lmax);
+  }
+  if (nan_seen) has_nan[my_c] = 1u;
+  __syncthreads();
+
+  const uint32_t r0 = (uint32_t)(nseg - 1) / 2;
+  const uint32_t kmin = kmin_s[col];
+  const uint32_t range = kmax_s[col] - kmin;
+  const int nbits = range ? 32 - __builtin_clz(range) : 0;  // bits of k - kmin
+  const uint32_t* keys = tile + col * stride;
+  uint32_t* h = hist + col * MED_HSTRIDE;
+  uint32_t prefix = 0u, rank = r0;  // prefix: bits >= hi of the selected k - kmin
+  // every column of the block runs the same number of passes (the block's
+  // largest), so the barriers are uniform; a column past its own bits is done
+  int hi_max = 0;
+  for (int c = 0; c < cpb; ++c) {
+    const uint32_t rc = kmax_s[c] - kmin_s[c];
+    const int nb = rc ? 32 - __builtin_clz(rc) : 0;
+    hi_max = nb > hi_max ? nb : hi_max;
+  }
+  int hi = nbits;
+  for (int hb = hi_max; hb > 0; hb -= MED_BITS) {
+    const int lo = hi - MED_BITS > 0 ? hi - MED_BITS : 0;
+    const int w = hi - lo;
+    const bool active = w > 0;
+    const int nb = 1 << w;
+    const int per = (MED_BUCKETS + tpc - 1) / tpc;
+    for (int b = lt; b < MED_HSTRIDE; b += tpc) h[b] = 0u;
+    __syncthreads();
+    if (active) {
+      const uint32_t dm = (uint32_t)nb - 1u;
+      for (int i = lt; i < nseg; i += tpc) {
+        const uint32_t r = keys[i] - kmin;
+        if ((uint32_t)((uint64_t)r >> hi) == prefix)
+          atomicAdd(&h[med_slot((int)((r >> lo) & dm))], 1u);
+      }
+    }
+    __syncthreads();
+    uint32_t sum = 0u;
+    for (int b = lt * per; b < (lt + 1) * per && b < MED_BUCKETS; ++b) sum += h[med_slot(b)];
+    part[tid] = sum;
+    __syncthreads();
+    if (active) {
+      uint32_t before = 0u;
+      for (int j = 0; j < lt; ++j) before += part[col * tpc + j];
+      if (rank >= before && rank < before + sum) {
+        uint32_t cum = before;
+        for (int b = lt * per; b < (lt + 1) * per && b < MED_BUCKETS; ++b) {
+          const uint32_t c = h[med_slot(b)];
+          if (rank < cum + c) {
+            sel_bucket[col] = (uint32_t)b;
+            sel_rank[col] = rank - cum;
+            break;
+          }
+          cum += c;
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+      prefix = (prefix << w) | sel_bucket[col];
+      rank = sel_rank[col];
+      hi = lo;
+    }
+    __syncthreads();  // sel_* read by every thread before the next pass writes them
+  }
+  const uint32_t k0 = kmin + prefix;
+  const bool even = (nseg & 1) == 0;
+  uint32_t k1 = k0;
+  if (even) {
+    uint32_t c = 0u, mn = 0xFFFFFFFFu;
+    for (int i = lt; i < nseg; i += tpc) {
+      const uint32_t k = keys[i];
+      if (k <= k0) ++c;
+      else mn = k < mn ? k : mn;
+    }
+    atomicAdd(&cnt_le[col], c);
+    atomicMin(&min_gt[col], mn);
+    __syncthreads();
+    k1 = cnt_le[col] > r0 + 1 ? k0 : min_gt[col];
+  }
+  const int gc = col0 + col;
+  if (lt == 0 && col < ncol) {
+    float m = even ? 0.5f * (med_val(k0) + med_val(k1)) : med_val(k0);
+    if (has_nan[col]) m = __uint_as_float(0x7FC00000u);
+    const int bin = gc / a.channels;
+    const float f = (a.apply_factor && !(bin == 0 || 2 * bin == a.nfft)) ? 2.f : 1.f;
+    as_global_mut(a.out)[row * a.cols + gc] = m * (a.scale * f);
+  }
+}
+
 // ---------------------------------------------------------------- dispatch
 static bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
@@ -884,6 +1089,29 @@ int msig_overlap_add(const float* frames, int64_t rows, int32_t nseg, int32_t np
   hipLaunchKernelGGL(k_overlap_add, grid, block, 0, static_cast<hipStream_t>(stream), o);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "overlap-add kernel launch");
+  return 0;
+}
+
+int msig_median_frames(const float* frames, int64_t rows, int32_t nseg, int32_t cols,
+                       int32_t channels, double scale, int32_t apply_factor, int32_t nfft,
+                       float* out, void* stream) {
+  if (!frames || !out) return fail(MSIG_EINVAL, "NULL pointer");
+  if (rows < 1 || nseg < 1 || cols < 1) return fail(MSIG_EINVAL, "rows, nseg and cols must be >= 1");
+  if (channels != 1 && channels != 2) return fail(MSIG_EINVAL, "channels must be 1 or 2");
+  if (nseg > MSIG_MEDIAN_MAX_NSEG)
+    return fail(MSIG_ENOTSUP, "nseg %d above the staged median's %d", nseg, MSIG_MEDIAN_MAX_NSEG);
+  int cpb = MED_MAX_CPB;
+  while (cpb > 1 && med_lds_bytes(cpb, med_stride(nseg, cpb)) > (size_t)MED_LDS_BUDGET) cpb /= 2;
+  const int stride = med_stride(nseg, cpb);
+  const size_t lds = med_lds_bytes(cpb, stride);
+  const int ncb = (cols + cpb - 1) / cpb;
+  if (rows * (int64_t)ncb > INT32_MAX) return fail(MSIG_EINVAL, "too many columns");
+  MedArgs m{frames, out, rows, nseg, cols, cpb, ncb, channels, apply_factor ? 1 : 0, nfft,
+            stride, (float)scale};
+  hipLaunchKernelGGL(k_median, dim3((unsigned)(rows * ncb)), dim3(MED_BT), lds,
+                     static_cast<hipStream_t>(stream), m);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "median kernel launch");
   return 0;
 }
 

@@ -111,6 +111,7 @@ SIGNATURES = {
     "msig_batch_destroy": (ctypes.c_int, [_vp]),
     "msig_istft_frames": (ctypes.c_int, [_vp, _i64, _i32, _i32, _vp, _vp, _vp]),
     "msig_overlap_add": (ctypes.c_int, [_vp, _i64, _i32, _i32, _i32, _i32, _vp, _i64, _i64, _vp, _vp]),
+    "msig_median_frames": (ctypes.c_int, [_vp, _i64, _i32, _i32, _i32, ctypes.c_double, _i32, _i32, _vp, _vp]),
     "msig_kernel_timing_arm": (ctypes.c_int, [_i32]),
     "msig_kernel_timing_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i32)]),
 }

@@ -375,3 +375,25 @@ def overlap_add(frames, win, nperseg: int, hop: int, first: int, count: int):
     _hip.check(rc, "overlap_add")
     launches["overlap_add"] += 1
     return torch.view_as_complex(out) if cplx else out
+
+
+MEDIAN_MAX_NSEG = 16000
+
+
+def median_frames(p, scale: float, apply_factor: bool, nfft: int):
+    """Median over the segment axis (numpy semantics) of (rows, nseg, bins)
+    float32 or complex64 frames (real and imaginary parts separately), times
+    ``scale`` and, with ``apply_factor``, the onesided factor of each bin
+    (reference: the median average of csd, spectral.py:469-477).  Returns
+    (rows, bins) of the input's dtype."""
+    rows, nseg, bins = p.shape
+    cplx = p.dtype == torch.complex64
+    src = torch.view_as_real(p.contiguous()).reshape(rows, nseg, 2 * bins) if cplx else p.contiguous()
+    cols = src.shape[-1]
+    out = torch.empty(rows, cols, dtype=torch.float32, device=p.device)
+    rc = _hip.load().msig_median_frames(_hip.ptr(src), rows, nseg, cols, 2 if cplx else 1,
+                                       float(scale), int(bool(apply_factor)), int(nfft),
+                                       _hip.ptr(out), _hip.stream_of(src))
+    _hip.check(rc, "median_frames")
+    launches["median_frames"] += 1
+    return torch.view_as_complex(out.reshape(rows, bins, 2)) if cplx else out
