@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define MSIG_ABI_VERSION 1
+#define MSIG_ABI_VERSION 2
 
 #define MSIG_EINVAL (-22)
 #define MSIG_ENOTSUP (-95)
@@ -176,13 +176,17 @@ typedef struct msig_batch_job {
 } msig_batch_job;
 
 /* Per FFT size (index log2(nfft)): 0 selects the default.
- *   max_run:       most segment pairs one thread group sums sequentially (<= 32)
- *   target_blocks: blocks a job's geometry aims for (default 1024)
- *   blocks_per_cu: resident blocks per CU of the persistent launch (default 4) */
+ *   max_run:        most segment pairs one thread group sums sequentially (<= 32)
+ *   target_blocks:  blocks a job's geometry aims for (default 1024)
+ *   blocks_per_cu:  resident blocks per CU of the persistent launch (default 4)
+ *   radix_schedule: 0 = radix-16 passes first, remainder radix last (default);
+ *                   1 = remainder radix first (nfft = 16^m r, r in {2, 4, 8});
+ *                   identical for powers of 16 */
 typedef struct msig_tuning {
   int32_t max_run[13];
   int32_t target_blocks[13];
   int32_t blocks_per_cu[13];
+  int32_t radix_schedule[13];
 } msig_tuning;
 
 typedef struct msig_batch msig_batch; /* opaque plan */

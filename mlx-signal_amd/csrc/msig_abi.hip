@@ -517,13 +517,14 @@ static hipError_t dispatch_istft(int N, const KArgs& a, int64_t rows, hipStream_
   return hipErrorInvalidValue;
 }
 
-static hipError_t dispatch_mean_batch(int N, int kk, bool cplx, bool half, const KArgs* jobs,
+static hipError_t dispatch_mean_batch(int N, int kk, bool cplx, bool half, int sched,
+                                      const KArgs* jobs,
                                       const WorkItem* queue, int32_t nwork, float* ws, int grid,
                                       hipStream_t s) {
   switch (N) {
 #define C(NV) \
   case NV:    \
-    return launch_mean_batch_##NV(kk, cplx, half, jobs, queue, nwork, ws, grid, s);
+    return launch_mean_batch_##NV(kk, cplx, half, sched, jobs, queue, nwork, ws, grid, s);
     C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
   }
@@ -645,7 +646,7 @@ static int log2i(int v) {
 // and the launch list of one run.  The device copy lives in caller memory.
 struct msig_batch {
   struct Class {
-    int N, kk;
+    int N, kk, sched;
     bool cplx, half;
     int32_t qoff, nwork;
     int grid;
@@ -950,6 +951,7 @@ int msig_batch_create(const msig_batch_job* jobs, int32_t njobs, const msig_tuni
     c.nwork = (int32_t)kv.second.size();
     int bpc = 4;
     if (tuning && tuning->blocks_per_cu[log2i(c.N)] > 0) bpc = tuning->blocks_per_cu[log2i(c.N)];
+    c.sched = (tuning && tuning->radix_schedule[log2i(c.N)] == 1) ? 1 : 0;
     const int64_t cap = (int64_t)bpc * cus;
     c.grid = (int)(c.nwork < cap ? c.nwork : cap);
     p->queue.insert(p->queue.end(), kv.second.begin(), kv.second.end());
@@ -1017,7 +1019,8 @@ int msig_batch_run(const msig_batch* plan, const void* device, void* workspace, 
   const WorkItem* finq = reinterpret_cast<const WorkItem*>(d + plan->off_finq);
   float* ws = static_cast<float*>(workspace);
   for (const auto& c : plan->classes) {
-    e = dispatch_mean_batch(c.N, c.kk, c.cplx, c.half, jobs, queue + c.qoff, c.nwork, ws, c.grid, s);
+    e = dispatch_mean_batch(c.N, c.kk, c.cplx, c.half, c.sched, jobs, queue + c.qoff, c.nwork, ws,
+                            c.grid, s);
     if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
   }
   hipLaunchKernelGGL(k_finalize_batch, dim3(plan->fin_grid), dim3(FIN_BINS * FIN_LANES), 0, s, fins,

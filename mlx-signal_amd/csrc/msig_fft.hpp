@@ -155,16 +155,30 @@ struct Dft {
   }
 };
 
-// Radix schedule for an N-point FFT carried E points per thread:
-// E, E, ..., remainder.  N and E are powers of two, E <= N.
-template <int N, int E>
+// Radix schedule for an N-point FFT carried E points per thread.  N = E^m r
+// with r < E; schedule S = 0 runs E, E, ..., r (remainder last), S = 1 runs
+// r, E, ..., E (remainder first).  When N is a power of E both are E, ..., E.
+// N and E are powers of two, E <= N.
+//
+// Pass 0 reads the thread's registers in place: register e holds position
+// t + TPF u + (N / R0) j (u = e / R0, j = e % R0, R0 = radix(0)), which is
+// sample t + TPF (u + (E / R0) j); load_order() maps register e to the
+// sample index the caller loaded into register u + (E / R0) j.
+template <int N, int E, int S = 0>
 struct Plan {
   static_assert((N & (N - 1)) == 0 && (E & (E - 1)) == 0 && E <= N, "pow2 sizes");
   static constexpr int TPF = N / E;  // threads per FFT
+  static constexpr int rem() {
+    int r = N;
+    while (r >= E) r /= E;
+    return r;  // 1 when N is a power of E
+  }
+  static constexpr bool REM_FIRST = S == 1 && rem() > 1;
   static constexpr int radix(int p) {
-    int rem = N;
-    for (int i = 0; i < p; ++i) rem /= (rem < E ? rem : E);
-    return rem < E ? rem : E;
+    if (REM_FIRST) return p == 0 ? rem() : E;
+    int r = N;
+    for (int i = 0; i < p; ++i) r /= (r < E ? r : E);
+    return r < E ? r : E;
   }
   static constexpr int npasses() {
     int rem = N, c = 0;
@@ -215,6 +229,10 @@ struct Plan {
     return t + TPF * u + (N / R) * j;
   }
   // twiddle exponent (units of exp(-2*pi*i/N)) for pass P, register e
+  // caller register feeding pass-0 register e (identity for S = 0)
+  static MSIG_HD constexpr int load_order(int e) {
+    return (e / radix(0)) + (E / radix(0)) * (e % radix(0));
+  }
   template <int P>
   static MSIG_HD int tw_exp(int t, int e) {
     constexpr int R = radix(P);
@@ -227,9 +245,9 @@ struct Plan {
 
 // Load the per-thread twiddle registers from an N-entry table of
 // exp(-2*pi*i*e/N) (computed in FP64 by the host, stored as float pairs).
-template <int N, int E, int P = 1, class TablePtr>
+template <int N, int E, int P = 1, class TablePtr, int S = 0>
 MSIG_HD void load_twiddles(int t, TablePtr table, cf* tw) {
-  using PL = Plan<N, E>;
+  using PL = Plan<N, E, S>;
   if constexpr (P < PL::NP) {
     constexpr int R = PL::radix(P);
     constexpr int off = PL::tw_off(P);
@@ -241,7 +259,7 @@ MSIG_HD void load_twiddles(int t, TablePtr table, cf* tw) {
         const int ex = PL::template tw_exp<P>(t, u * R + j);
         tw[off + u * (R - 1) + (j - 1)] = cf{table[ex].re, table[ex].im};
       }
-    load_twiddles<N, E, P + 1>(t, table, tw);
+    load_twiddles<N, E, P + 1, TablePtr, S>(t, table, tw);
   }
 }
 
@@ -249,9 +267,9 @@ MSIG_HD void load_twiddles(int t, TablePtr table, cf* tw) {
 // slot s of this thread is tw[s * TS]: TS = 1 for a per-thread register
 // array, TS = TPF for the slot-major LDS table (one column per thread
 // position, so every read is one ds_read_b64 at a constant offset).
-template <int N, int E, int P, int TS = 1>
+template <int N, int E, int P, int TS = 1, int S = 0>
 MSIG_HD void pass_compute(cf* v, const cf* tw) {
-  using PL = Plan<N, E>;
+  using PL = Plan<N, E, S>;
   constexpr int R = PL::radix(P);
 #pragma unroll
   for (int u = 0; u < E / R; ++u) {
@@ -265,16 +283,16 @@ MSIG_HD void pass_compute(cf* v, const cf* tw) {
   }
 }
 
-template <int N, int E, int P>
+template <int N, int E, int P, int S = 0>
 MSIG_HD void pass_store(int t, const cf* v, cf* lds) {
-  using PL = Plan<N, E>;
+  using PL = Plan<N, E, S>;
 #pragma unroll
   for (int e = 0; e < E; ++e) lds[PL::pad(PL::template out_pos<P>(t, e))] = v[e];
 }
 
-template <int N, int E, int P>
+template <int N, int E, int P, int S = 0>
 MSIG_HD void pass_load(int t, cf* v, const cf* lds) {
-  using PL = Plan<N, E>;
+  using PL = Plan<N, E, S>;
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = lds[PL::pad(PL::template in_pos<P>(t, e))];
 }

@@ -113,9 +113,9 @@ __device__ __forceinline__ void group_sum2(float& a, float& b, float* red) {
 // Where a kernel keeps its Stockham twiddles: in an LDS table (slot-major,
 // TWN x TPF) when that table is small, else in registers.  The LDS table
 // frees ~2*TWN VGPRs per thread, which buys occupancy.
-template <int N, int E>
+template <int N, int E, int S = 0>
 struct TwiddleHome {
-  using PL = Plan<N, E>;
+  using PL = Plan<N, E, S>;
   static constexpr bool LDS = (PL::NP > 1) && (PL::TWN * PL::TPF * 8 <= 32768);
   static constexpr int LDS_ELEMS = LDS ? PL::TWN * PL::TPF : 1;
   static constexpr int REGS = LDS ? 1 : PL::TWN;
@@ -123,30 +123,40 @@ struct TwiddleHome {
 };
 
 // Fill the slot-major LDS twiddle table from the global FP64-derived table.
-template <int N, int E>
+template <int N, int E, int S = 0>
 __device__ __forceinline__ void fill_twiddle_lds(cf* twl, const cf* table) {
-  using PL = Plan<N, E>;
+  using PL = Plan<N, E, S>;
   for (int t = threadIdx.x; t < PL::TPF; t += BT) {
     cf tmp[PL::TWN];
-    load_twiddles<N, E>(t, as_global(table), tmp);
+    load_twiddles<N, E, 1, const MSIG_GLOBAL cf*, S>(t, as_global(table), tmp);
 #pragma unroll
     for (int s = 0; s < PL::TWN; ++s) twl[s * PL::TPF + t] = tmp[s];
   }
 }
 
-// Forward FFT of the group's N points held E per thread; tw points at this
-// thread's twiddle slot 0 (stride TS between slots).
-template <int N, int E, int TS, int P = 1>
+// Forward FFT of the group's N points held E per thread (register e holds
+// sample t + TPF e); tw points at this thread's twiddle slot 0 (stride TS
+// between slots); radix schedule S (Plan).
+template <int N, int E, int TS, int P = 1, int S = 0>
 __device__ __forceinline__ void fft_passes(int t, cf* v, const cf* tw, cf* lds) {
-  using PL = Plan<N, E>;
-  if constexpr (P == 1) pass_compute<N, E, 0, TS>(v, tw);
+  using PL = Plan<N, E, S>;
+  if constexpr (P == 1) {
+    if constexpr (PL::REM_FIRST) {  // pass-0 register order (a renaming)
+      cf tmp[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) tmp[e] = v[PL::load_order(e)];
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = tmp[e];
+    }
+    pass_compute<N, E, 0, TS, S>(v, tw);
+  }
   if constexpr (P < PL::NP) {
-    pass_store<N, E, P - 1>(t, v, lds);
+    pass_store<N, E, P - 1, S>(t, v, lds);
     group_sync<PL::TPF>();
-    pass_load<N, E, P>(t, v, lds);
+    pass_load<N, E, P, S>(t, v, lds);
     if constexpr (PL::TPF > 64) __syncthreads();  // reads done before next stores
-    pass_compute<N, E, P, TS>(v, tw);
-    fft_passes<N, E, TS, P + 1>(t, v, tw, lds);
+    pass_compute<N, E, P, TS, S>(v, tw);
+    fft_passes<N, E, TS, P + 1, S>(t, v, tw, lds);
   }
 }
 
@@ -182,10 +192,10 @@ __device__ __forceinline__ void prep_unit(cf* z, const float* w, int detrend, in
 
 // Grouped mean over segments: per-bin sums of |Zx|^2 (PSD), conj(Zx) Zy
 // (CSD) or both spectra and the cross term (COH), written per block.
-template <int N, int E, int KIND, bool CPLX, bool HALF>
+template <int N, int E, int KIND, bool CPLX, bool HALF, int S = 0>
 __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, cf* lds_all,
                                           float* red, const cf* twl, float* ws) {
-  using PL = Plan<N, E>;
+  using PL = Plan<N, E, S>;
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
   constexpr int NC = KIND == K_PSD ? 1 : (KIND == K_CSD ? 2 : 4);
@@ -194,13 +204,13 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
   cf* lds = lds_all + g * PL::LDS;
   const int unit0 = (bir * GPB + g) * a.run;
 
-  using TH = TwiddleHome<N, E>;
+  using TH = TwiddleHome<N, E, S>;
   cf twr[TH::REGS];
   const cf* tw;
   if constexpr (TH::LDS) {
     tw = twl + t;
   } else {
-    load_twiddles<N, E>(t, as_global(a.twt), twr);
+    load_twiddles<N, E, 1, const MSIG_GLOBAL cf*, S>(t, as_global(a.twt), twr);
     tw = twr;
   }
   float w[E];
@@ -228,17 +238,17 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
   auto unit_body = [&](cf (&zx)[E], cf (&zy)[E], bool active) {
     if constexpr (KIND == K_PSD) {
       prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS>(t, zx, tw, lds);
+      fft_passes<N, E, TH::TS, 1, S>(t, zx, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) acc[0][e] = fmaf(zx[e].im, zx[e].im, fmaf(zx[e].re, zx[e].re, acc[0][e]));
       }
     } else {
       prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS>(t, zx, tw, lds);
+      fft_passes<N, E, TH::TS, 1, S>(t, zx, tw, lds);
       group_sync<TPF>();  // last-pass reads of zx done before zy's stores
       prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS>(t, zy, tw, lds);
+      fft_passes<N, E, TH::TS, 1, S>(t, zy, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -307,7 +317,7 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
     if constexpr (KIND == K_PSD) {
       if (more) sx.fetch(unit + 1, false, a);
       prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS>(t, zx, tw, lds);
+      fft_passes<N, E, TH::TS, 1, S>(t, zx, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) acc[0][e] = fmaf(zx[e].im, zx[e].im, fmaf(zx[e].re, zx[e].re, acc[0][e]));
@@ -320,10 +330,10 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
         sy.fetch(unit + 1, false, a);
       }
       prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS>(t, zx, tw, lds);
+      fft_passes<N, E, TH::TS, 1, S>(t, zx, tw, lds);
       group_sync<TPF>();  // last-pass reads of zx done before zy's stores
       prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS>(t, zy, tw, lds);
+      fft_passes<N, E, TH::TS, 1, S>(t, zy, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -662,31 +672,31 @@ constexpr int batch_waves() {
   return mean_waves<N, KIND>();
 }
 
-template <int N, int E, int KIND, bool CPLX, bool HALF>
+template <int N, int E, int KIND, bool CPLX, bool HALF, int S>
 __global__ __launch_bounds__(BT, (batch_waves<N, KIND>())) void k_mean_batch(
     const KArgs* __restrict__ jobs, const WorkItem* __restrict__ queue, int32_t nwork,
     float* ws_base) {
   using PL = Plan<N, E>;
   __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
   __shared__ float red[2 * (BT / 64)];
-  __shared__ cf twl[TwiddleHome<N, E>::LDS_ELEMS];
+  __shared__ cf twl[TwiddleHome<N, E, S>::LDS_ELEMS];
   bool filled = false;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const WorkItem item = queue[w];
     const KArgs& a = jobs[item.job];
     float* ws = ws_base + a.ws_off;
-    if constexpr (TwiddleHome<N, E>::LDS) {
+    if constexpr (TwiddleHome<N, E, S>::LDS) {
       if (!filled) {  // every job of a class has the same N: fill once
-        fill_twiddle_lds<N, E>(twl, a.twt);
+        fill_twiddle_lds<N, E, S>(twl, a.twt);
         filled = true;
       }
     }
     __syncthreads();
     if constexpr (HALF) {
-      mean_body<N, E, KIND, false, true>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl,
+      mean_body<N, E, KIND, false, true, S>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl,
                                          ws);
     } else {
-      mean_body<N, E, KIND, CPLX, false>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl,
+      mean_body<N, E, KIND, CPLX, false, S>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl,
                                          ws);
     }
     __syncthreads();

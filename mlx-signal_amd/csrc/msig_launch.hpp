@@ -67,9 +67,9 @@ hipError_t launch_mean_mfma256(int prec, const KArgs& a, int64_t rows, hipStream
                               hipStream_t s);                                              \
   hipError_t launch_frames_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows, \
                                 hipStream_t s);                                             \
-  hipError_t launch_mean_batch_##NV(int kind, bool cplx, bool half, const KArgs* jobs,      \
-                                    const WorkItem* queue, int32_t nwork, float* ws,        \
-                                    int grid, hipStream_t s);                               \
+  hipError_t launch_mean_batch_##NV(int kind, bool cplx, bool half, int sched,             \
+                                    const KArgs* jobs, const WorkItem* queue, int32_t nwork, \
+                                    float* ws, int grid, hipStream_t s);                    \
   hipError_t launch_istft_##NV(const KArgs& a, int64_t rows, hipStream_t s);
 MSIG_DECLARE_LAUNCHERS(16)
 MSIG_DECLARE_LAUNCHERS(32)
@@ -90,9 +90,18 @@ MSIG_DECLARE_LAUNCHERS(4096)
   hipLaunchKernelGGL((k_mean<NV, (NV < 16 ? NV : 16), K, C, H>), grid, block, 0, s, a)
 #define MSIG_LAUNCH_FRAMES(NV, K, C, H) \
   hipLaunchKernelGGL((k_frames<NV, (NV < 16 ? NV : 16), K, C, H>), grid, block, 0, s, a)
+#define MSIG_LAUNCH_MEAN_BATCH_S(NV, K, C, H, SCH)                                   \
+  hipLaunchKernelGGL((k_mean_batch<NV, (NV < 16 ? NV : 16), K, C, H, SCH>), dim3(grid),  \
+                     dim3(BT), 0, s, jobs, queue, nwork, ws)
+// radix schedule 1 is a distinct kernel only where N is not a power of 16
 #define MSIG_LAUNCH_MEAN_BATCH(NV, K, C, H)                                          \
-  hipLaunchKernelGGL((k_mean_batch<NV, (NV < 16 ? NV : 16), K, C, H>), dim3(grid), dim3(BT), \
-                     0, s, jobs, queue, nwork, ws)
+  do {                                                                               \
+    constexpr bool REM = Plan<NV, (NV < 16 ? NV : 16), 1>::REM_FIRST;                \
+    if (sched == 1 && REM)                                                           \
+      MSIG_LAUNCH_MEAN_BATCH_S(NV, K, C, H, (REM ? 1 : 0));                          \
+    else                                                                             \
+      MSIG_LAUNCH_MEAN_BATCH_S(NV, K, C, H, 0);                                      \
+  } while (0)
 #define MSIG_KIND_SWITCH(L, NV, K0, K1, K2, C, H) \
   if (kind == K0) L(NV, K0, C, H);                \
   else if (kind == K1) L(NV, K1, C, H);           \
@@ -127,9 +136,9 @@ MSIG_DECLARE_LAUNCHERS(4096)
     }                                                                                      \
     return hipGetLastError();                                                              \
   }                                                                                        \
-  hipError_t launch_mean_batch_##NV(int kind, bool cplx, bool half, const KArgs* jobs,      \
-                                    const WorkItem* queue, int32_t nwork, float* ws,        \
-                                    int grid, hipStream_t s) {                              \
+  hipError_t launch_mean_batch_##NV(int kind, bool cplx, bool half, int sched,             \
+                                    const KArgs* jobs, const WorkItem* queue, int32_t nwork, \
+                                    float* ws, int grid, hipStream_t s) {                   \
     if (cplx) {                                                                            \
       MSIG_KIND_SWITCH(MSIG_LAUNCH_MEAN_BATCH, NV, K_PSD, K_CSD, K_COH, true, false)       \
     } else if (half) {                                                                     \

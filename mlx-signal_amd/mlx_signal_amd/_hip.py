@@ -17,7 +17,7 @@ import os
 
 # MSIG_LIBRARY selects another build of the library (e.g. a tuning variant)
 LIB_PATH = Path(os.environ.get("MSIG_LIBRARY") or Path(__file__).with_name("libmsig.so"))
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 MSIG_EINVAL = -22
 MSIG_ENOTSUP = -95
@@ -75,6 +75,7 @@ class MsigTuning(ctypes.Structure):
         ("max_run", ctypes.c_int32 * 13),
         ("target_blocks", ctypes.c_int32 * 13),
         ("blocks_per_cu", ctypes.c_int32 * 13),
+        ("radix_schedule", ctypes.c_int32 * 13),
     ]
 
 

@@ -50,7 +50,8 @@ class SpectralRequest:
 
 
 def tuning_struct(tuning) -> _hip.MsigTuning | None:
-    """{nfft: {"max_run", "target_blocks", "blocks_per_cu"}} -> MsigTuning."""
+    """{nfft: {"max_run", "target_blocks", "blocks_per_cu", "radix_schedule"}}
+    -> MsigTuning."""
     if not tuning:
         return None
     t = _hip.MsigTuning()
@@ -59,6 +60,7 @@ def tuning_struct(tuning) -> _hip.MsigTuning | None:
         t.max_run[l2] = int(knobs.get("max_run", 0))
         t.target_blocks[l2] = int(knobs.get("target_blocks", 0))
         t.blocks_per_cu[l2] = int(knobs.get("blocks_per_cu", 0))
+        t.radix_schedule[l2] = int(knobs.get("radix_schedule", 0))
     return t
 
 

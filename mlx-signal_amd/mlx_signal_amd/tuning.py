@@ -9,7 +9,10 @@ the tile shape of a job's decomposition:
   segments are split across blocks);
 * ``max_run``       - segment pairs one thread group sums before its partial
   is written (run length of the register accumulation);
-* ``blocks_per_cu`` - resident blocks per CU of the persistent launch.
+* ``blocks_per_cu`` - resident blocks per CU of the persistent launch;
+* ``radix_schedule`` - order of the FFT passes: 0 runs the radix-16 passes
+  first and the remainder radix last, 1 the remainder radix first (a
+  separate kernel only where nfft is not a power of 16).
 
 The chosen table is cached per device name in a JSON file so later processes
 reuse it.  Jobs are planned with their size's tuned knobs both alone and in a
@@ -28,12 +31,25 @@ import torch
 
 from .batch import SpectralBatch, SpectralRequest
 
-__all__ = ["autotune", "cached_tuning", "load_tuning", "save_tuning", "CANDIDATES"]
+__all__ = ["autotune", "cached_tuning", "candidates_for", "load_tuning", "save_tuning", "CANDIDATES"]
 
 CANDIDATES = [
     dict(target_blocks=tb, max_run=mr, blocks_per_cu=bpc)
     for tb, mr, bpc in itertools.product((256, 1024, 4096), (8, 32), (2, 4, 8))
 ]
+
+
+def _schedules(nfft: int):
+    """Radix schedules that differ for this size (nfft = 16^m r, r > 1)."""
+    r = int(nfft)
+    while r >= 16:
+        r //= 16
+    return (0, 1) if r > 1 else (0,)
+
+
+def candidates_for(nfft: int, base=None):
+    """Tile-shape candidates crossed with the size's distinct radix schedules."""
+    return [dict(c, radix_schedule=sch) for c in (base or CANDIDATES) for sch in _schedules(nfft)]
 
 
 def _cache_path() -> Path:
@@ -111,7 +127,7 @@ def autotune(nffts=(256, 1024, 4096), reps=5, seed=0, candidates=None, save=True
     for nfft in nffts:
         reqs = _probe_requests(nfft, dev, gen)
         timings = []
-        for cand in candidates or CANDIDATES:
+        for cand in candidates_for(nfft, candidates):
             b = SpectralBatch(reqs, tuning={nfft: cand})
             timings.append((_time_batch(b, reps), cand))
             b.close()

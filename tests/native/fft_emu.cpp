@@ -37,15 +37,15 @@ static std::vector<cd> dft(const std::vector<cd>& z) {
   return out;
 }
 
-template <int N, int E, int P>
+template <int N, int E, int P, int S = 0>
 static bool run_pass(std::vector<std::vector<cf>>& v, const std::vector<std::vector<cf>>& tw,
                      std::vector<cf>& lds, std::vector<int>& wcount) {
-  using PL = msig::Plan<N, E>;
+  using PL = msig::Plan<N, E, S>;
   constexpr int TPF = PL::TPF;
   if constexpr (P < PL::NP) {
     std::fill(wcount.begin(), wcount.end(), 0);
     for (int t = 0; t < TPF; ++t) {
-      msig::pass_store<N, E, P - 1>(t, v[t].data(), lds.data());
+      msig::pass_store<N, E, P - 1, S>(t, v[t].data(), lds.data());
       for (int e = 0; e < E; ++e) wcount[PL::pad(PL::template out_pos<P - 1>(t, e))]++;
     }
     for (int i = 0; i < (int)wcount.size(); ++i)
@@ -61,28 +61,37 @@ static bool run_pass(std::vector<std::vector<cf>>& v, const std::vector<std::vec
           return false;
         }
       }
-      msig::pass_load<N, E, P>(t, v[t].data(), lds.data());
+      msig::pass_load<N, E, P, S>(t, v[t].data(), lds.data());
     }
-    for (int t = 0; t < TPF; ++t) msig::pass_compute<N, E, P>(v[t].data(), tw[t].data());
-    return run_pass<N, E, P + 1>(v, tw, lds, wcount);
+    for (int t = 0; t < TPF; ++t) msig::pass_compute<N, E, P, 1, S>(v[t].data(), tw[t].data());
+    return run_pass<N, E, P + 1, S>(v, tw, lds, wcount);
   }
   return true;
 }
 
-// FFT of z (N complex points) exactly as a group of TPF device threads does it.
-template <int N, int E>
+// FFT of z (N complex points) exactly as a group of TPF device threads does it
+// (radix schedule S; register e of a thread starts with sample t + TPF e and
+// is renamed to pass 0's order as fft_passes does).
+template <int N, int E, int S = 0>
 static bool emu_fft(const std::vector<cf>& table, const std::vector<cf>& z, std::vector<cf>& Z) {
-  using PL = msig::Plan<N, E>;
+  using PL = msig::Plan<N, E, S>;
   constexpr int TPF = PL::TPF;
   std::vector<std::vector<cf>> v(TPF, std::vector<cf>(E)), tw(TPF, std::vector<cf>(PL::TWN));
   for (int t = 0; t < TPF; ++t) {
-    msig::load_twiddles<N, E>(t, table.data(), tw[t].data());
-    for (int e = 0; e < E; ++e) v[t][e] = z[t + TPF * e];
-    msig::pass_compute<N, E, 0>(v[t].data(), tw[t].data());
+    msig::load_twiddles<N, E, 1, const cf*, S>(t, table.data(), tw[t].data());
+    for (int e = 0; e < E; ++e) {
+      const int src = PL::REM_FIRST ? PL::load_order(e) : e;
+      if (PL::template in_pos<0>(t, e) != t + TPF * src) {
+        std::printf("N=%d S=%d: pass-0 register %d does not hold sample t + TPF * %d\n", N, S, e, src);
+        return false;
+      }
+      v[t][e] = z[t + TPF * src];
+    }
+    msig::pass_compute<N, E, 0, 1, S>(v[t].data(), tw[t].data());
   }
   std::vector<cf> lds(PL::LDS);
   std::vector<int> wcount(PL::LDS);
-  if (!run_pass<N, E, 1>(v, tw, lds, wcount)) return false;
+  if (!run_pass<N, E, 1, S>(v, tw, lds, wcount)) return false;
   Z.assign(N, cf{NAN, NAN});
   std::vector<int> seen(N, 0);
   for (int t = 0; t < TPF; ++t)
@@ -358,6 +367,37 @@ static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detren
   return ok ? 0 : 1;
 }
 
+// Radix schedule 1 (remainder radix first) against the FP64 DFT.
+template <int N>
+static int check_schedule(std::mt19937_64& rng) {
+  constexpr int E = N < 16 ? N : 16;
+  static_assert(msig::Plan<N, E, 1>::REM_FIRST, "a size where the schedules differ");
+  std::vector<cf> table(N);
+  for (int i = 0; i < N; ++i) {
+    const double a = -2.0 * M_PI * i / N;
+    table[i] = cf{(float)std::cos(a), (float)std::sin(a)};
+  }
+  std::normal_distribution<double> nd;
+  std::vector<cf> z(N), Z;
+  std::vector<cd> zd(N);
+  for (int i = 0; i < N; ++i) {
+    z[i] = cf{(float)nd(rng), (float)nd(rng)};
+    zd[i] = cd(z[i].re, z[i].im);
+  }
+  if (!emu_fft<N, E, 1>(table, z, Z)) return 1;
+  auto ref = dft(zd);
+  double peak = 0, err = 0;
+  for (int k = 0; k < N; ++k) {
+    peak = std::max(peak, std::abs(ref[k]));
+    err = std::max(err, std::abs(cd(Z[k].re, Z[k].im) - ref[k]));
+  }
+  const double rel = err / peak;
+  const bool ok = rel < 2e-6;
+  std::printf("schedule 1 N=%5d radices %d first, passes=%d fft_rel=%.3e %s\n", N,
+              msig::Plan<N, E, 1>::radix(0), msig::Plan<N, E, 1>::NP, rel, ok ? "ok" : "FAIL");
+  return ok ? 0 : 1;
+}
+
 int main() {
   std::mt19937_64 rng(1234);
   int bad = 0;
@@ -388,5 +428,12 @@ int main() {
   bad += check_half_groups<256>(rng, 9, 1, 1);
   bad += check_half_groups<1024>(rng, 13, 2, 1);
   bad += check_half_groups<1024>(rng, 1, 4, 1);
+  // radix schedule 1 wherever it differs from schedule 0
+  bad += check_schedule<32>(rng);
+  bad += check_schedule<64>(rng);
+  bad += check_schedule<128>(rng);
+  bad += check_schedule<512>(rng);
+  bad += check_schedule<1024>(rng);
+  bad += check_schedule<2048>(rng);
   return bad ? 1 : 0;
 }

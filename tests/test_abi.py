@@ -73,3 +73,16 @@ def test_kernel_timing_contract_without_gpu():
     assert lib.msig_kernel_timing_read(ctypes.byref(tot), ctypes.byref(n)) == 0
     assert (tot.value, n.value) == (0.0, 0)
     assert lib.msig_kernel_timing_read(None, ctypes.byref(n)) == _hip.MSIG_EINVAL
+
+
+def test_tuning_struct_matches_header():
+    """msig_tuning's fields, in order, are the ctypes structure's."""
+    import re
+
+    text = (ROOT / "include" / "msig.h").read_text()
+    body = re.search(r"typedef struct msig_tuning \{(.*?)\} msig_tuning;", text, re.S).group(1)
+    fields = re.findall(r"int32_t (\w+)\[13\];", body)
+    assert fields == [f for f, _ in _hip.MsigTuning._fields_]
+    assert ctypes.sizeof(_hip.MsigTuning) == 4 * 13 * len(fields)
+    abi = int(re.search(r"#define MSIG_ABI_VERSION (\d+)", text).group(1))
+    assert abi == _hip.ABI_VERSION

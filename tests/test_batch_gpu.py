@@ -66,7 +66,10 @@ def test_batch_matches_oracle_and_standalone(rng):
 
 @pytest.mark.parametrize("tuning", [None, {256: {"target_blocks": 64, "max_run": 4, "blocks_per_cu": 2},
                                           1024: {"target_blocks": 128, "max_run": 8},
-                                          4096: {"blocks_per_cu": 1}}])
+                                          4096: {"blocks_per_cu": 1}},
+                                    {64: {"radix_schedule": 1}, 128: {"radix_schedule": 1},
+                                     512: {"radix_schedule": 1},
+                                     1024: {"radix_schedule": 1, "max_run": 8}}])
 def test_batch_alone_equals_in_mix(rng, tuning):
     dev = torch.device("cuda")
     reqs, _ = _requests(rng, dev)
@@ -106,3 +109,14 @@ def test_batch_rejects_ineligible(rng):
         SpectralBatch([SpectralRequest("welch", x, nperseg=256, detrend="linear")])
     with pytest.raises(ValueError):
         SpectralBatch([SpectralRequest("csd", x, nperseg=256)])  # y missing
+
+
+def test_batch_radix_schedule_matches_oracle(rng):
+    """Remainder-radix-first kernels (nfft 64, 128, 512, 1024) against the
+    oracle; alone-equals-in-mix under such a tuning is in the test above."""
+    dev = torch.device("cuda")
+    reqs, host = _requests(rng, dev)
+    tuning = {n: {"radix_schedule": 1} for n in (64, 128, 512, 1024)}
+    outs = [o.clone() for o in SpectralBatch(reqs, tuning=tuning).run()]
+    for (kind, x, y, kw), out in zip(host, outs):
+        assert_peak_close(out, _oracle(kind, x, y, kw), TOL)
