@@ -13,6 +13,7 @@ import warnings
 import numpy as np
 import torch
 
+from . import _ingest
 from ._config import DowncastWarning, _config
 
 
@@ -56,15 +57,16 @@ def to_device(x, device: torch.device) -> torch.Tensor:
         elif a.dtype.kind in "fiub":
             a = a.astype(np.float32)
         return to_device(a, device)
-    if x.dtype == np.float32 or x.dtype == np.complex64:
-        return torch.from_numpy(np.ascontiguousarray(x)).to(device)
-    if x.dtype.kind == "f":
-        if x.dtype.itemsize > 4:
+    if x.dtype.kind in "fc":
+        cplx = x.dtype.kind == "c"
+        if (cplx and x.dtype != np.complex64) or (not cplx and x.dtype.itemsize > 4):
             _downcast_notice(str(x.dtype))
-        return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)
-    if x.dtype.kind == "c":
-        _downcast_notice(str(x.dtype))
-        return torch.from_numpy(np.ascontiguousarray(x, dtype=np.complex64)).to(device)
+        out_dtype = torch.complex64 if cplx else torch.float32
+        if device.type == "cuda" and x.size * (8 if cplx else 4) >= _ingest.MIN_BYTES:
+            # pipelined pinned upload; the downcast happens inside its copy
+            return _ingest.upload(x, out_dtype, device)
+        np_dtype = np.complex64 if cplx else np.float32
+        return torch.from_numpy(np.ascontiguousarray(x, dtype=np_dtype)).to(device)
     if x.dtype.kind in "iub":
         return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)
     raise TypeError(f"unsupported input dtype: {x.dtype}")
