@@ -519,6 +519,28 @@ def main():
             w, e, _, _ = time_steps(torch, fn, xs16, max(5, args.steps // 2), 2)
             secondary[name] = {"ms_event": e, "ms_wall": w, "GBps": nbytes / (e * 1e-3) / 1e9,
                                "samples_per_s": 16 * NS / (e * 1e-3)}
+        # istft of the 16 x 2^20 stft (fused inverse + gather overlap-add)
+        zs = [msig.stft(b, nperseg=1024)[2] for b in xs16]
+        nseg_i = zs[0].shape[-1]
+        w, e, _, _ = time_steps(torch, lambda z: msig.istft(z, nperseg=1024), zs,
+                                max(5, args.steps // 2), 2)
+        xr = msig.istft(zs[0], nperseg=1024)[1]
+        rt = float((xr[:, :NS] - xs16[0]).abs().max() / xs16[0].abs().max())
+        secondary["istft_16x2^20"] = {
+            "ms_event": e, "ms_wall": w,
+            "GBps": (8 * 16 * nseg_i * bins + 4 * 16 * NS) / (e * 1e-3) / 1e9,
+            "samples_per_s": 16 * NS / (e * 1e-3), "round_trip_max_err_vs_peak": rt}
+        del zs, xr
+        # welch median average (fused power frames + the per-bin median kernel)
+        w, e, _, _ = time_steps(
+            torch, lambda b: msig.welch(b, fs=FS, nperseg=NPERSEG, noverlap=NOVERLAP,
+                                        average="median"), xs, max(5, args.steps // 2), 2)
+        pm = msig.welch(xs[0][:2], fs=FS, nperseg=NPERSEG, noverlap=NOVERLAP, average="median")[1]
+        refm = oracle.welch(xs[0][:2].cpu().numpy(), fs=FS, nperseg=NPERSEG, noverlap=NOVERLAP,
+                            average="median")[1]
+        secondary["welch_median_64x2^20"] = {
+            "ms_event": e, "ms_wall": w, "samples_per_s": C * NS / (e * 1e-3),
+            "max_err_vs_fp64_peak": float(np.abs(pm.cpu().numpy() - refm).max() / np.abs(refm).max())}
         # end-to-end: NumPy in -> tensor -> host out (the reference's "e2e" column)
         xh = xs[0].cpu().numpy()
         ts = []
