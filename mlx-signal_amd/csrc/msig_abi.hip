@@ -274,32 +274,33 @@ __global__ __launch_bounds__(256) void k_overlap_add(OlaArgs o) {
 // factor of the column's bin.
 //
 // A block stages CPB columns of one row in LDS as order-preserving uint32
-// keys: one read of the frames, a thread per segment so every LDS write
-// instruction covers consecutive addresses.  Each column's key range
-// [kmin, kmax] is found first; the rank (nseg - 1) / 2 key is then selected
-// by radix passes over the range-relative keys k - kmin, up to 11 bits per
-// pass with LDS-atomic histograms.  Digits cut from the column's own range
-// spread its keys over the buckets (a column of power values spans a few
-// binary exponents only, which fixed top-bit digits would crowd into a few
-// buckets), and a constant column needs no pass.  For an even count one
-// more pass finds the next order statistic.  Counterpart of the median
-// average of csd (spectral.py:469-477, mx.median over the segment axis).
+// keys (one read of the frames, eight loads in flight per thread, the
+// column stride chosen so a wave's staging writes hit 64 banks) and takes
+// each column's key range [kmin, kmax] on the way.  The rank (nseg - 1) / 2
+// key is then selected by radix passes over the range-relative keys
+// k - kmin, 8 bits per pass, counting into LDS-atomic histograms: digits cut
+// from the column's own range spread its keys over the buckets (a column of
+// power values spans a few binary exponents only), a constant column needs
+// no pass, and only the first pass touches every key.  Each pass locates the
+// bucket holding the rank with a log-step scan of the column's per-lane
+// bucket sums.  For an even count one more pass finds the next order
+// statistic.  Counterpart of the median average of csd (spectral.py:469-477,
+// mx.median over the segment axis).
 constexpr int MED_BT = 256;
 constexpr int MED_MAX_CPB = 16;
-constexpr int MED_BITS = 11;
+constexpr int MED_BITS = 8;
 constexpr int MED_BUCKETS = 1 << MED_BITS;
-// histogram slot of bucket b: one pad word per 32 buckets, so the per-lane
-// sums over contiguous bucket ranges (lane l reads l * per + k) hit
-// distinct banks
+// histogram slot of bucket b: one pad word per 32 buckets, so per-lane sums
+// over contiguous bucket ranges hit distinct banks
 constexpr int MED_HSTRIDE = MED_BUCKETS + MED_BUCKETS / 32;
 __device__ __forceinline__ int med_slot(int b) { return b + (b >> 5); }
-constexpr int MED_LDS_BUDGET = 80 << 10;   // bytes per block: two blocks per CU
+constexpr int MED_LDS_BUDGET = 40 << 10;   // bytes per block: four blocks per CU
 
 struct MedArgs {
   const float* in;
   float* out;
   int64_t rows;
-  int32_t nseg, cols, cpb, ncb, channels, apply_factor, nfft, stride;
+  int32_t nseg, cols, ncb, channels, apply_factor, nfft, stride;
   float scale;
 };
 
@@ -324,12 +325,15 @@ __device__ __forceinline__ float med_val(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k ^ 0x80000000u) : ~k);
 }
 
+template <int CPB>
 __global__ __launch_bounds__(MED_BT) void k_median(MedArgs a) {
   extern __shared__ uint32_t med_smem[];
-  const int cpb = a.cpb, tpc = MED_BT / cpb, nseg = a.nseg, stride = a.stride;
-  uint32_t* tile = med_smem;                                   // [cpb][stride]
-  uint32_t* hist = tile + cpb * stride;                        // [cpb][MED_HSTRIDE]
-  uint32_t* part = hist + cpb * MED_HSTRIDE;                   // [MED_BT]
+  constexpr int TPC = MED_BT / CPB;          // threads per column
+  constexpr int PER = MED_BUCKETS / TPC;     // buckets each thread sums (>= 1)
+  const int nseg = a.nseg, stride = a.stride;
+  uint32_t* tile = med_smem;                                   // [CPB][stride]
+  uint32_t* hist = tile + CPB * stride;                        // [CPB][MED_HSTRIDE]
+  uint32_t* part = hist + CPB * MED_HSTRIDE;                   // [MED_BT]
   uint32_t* sel_bucket = part + MED_BT;                        // [MED_MAX_CPB] x 7
   uint32_t* sel_rank = sel_bucket + MED_MAX_CPB;
   uint32_t* cnt_le = sel_rank + MED_MAX_CPB;
@@ -338,10 +342,10 @@ __global__ __launch_bounds__(MED_BT) void k_median(MedArgs a) {
   uint32_t* kmin_s = has_nan + MED_MAX_CPB;
   uint32_t* kmax_s = kmin_s + MED_MAX_CPB;
   const int tid = threadIdx.x;
-  const int col = tid / tpc, lt = tid - col * tpc;
+  const int col = tid / TPC, lt = tid % TPC;
   const int64_t row = blockIdx.x / a.ncb;
-  const int col0 = (int)(blockIdx.x - row * a.ncb) * cpb;
-  const int ncol = a.cols - col0 < cpb ? a.cols - col0 : cpb;  // columns present
+  const int col0 = (int)(blockIdx.x - row * a.ncb) * CPB;
+  const int ncol = a.cols - col0 < CPB ? a.cols - col0 : CPB;  // columns present
   const MSIG_GLOBAL float* src = as_global(a.in) + row * (int64_t)nseg * a.cols + col0;
 
   if (tid < MED_MAX_CPB) {
@@ -352,12 +356,11 @@ __global__ __launch_bounds__(MED_BT) void k_median(MedArgs a) {
     kmax_s[tid] = 0u;
   }
   __syncthreads();
-  // staging: element i = (segment i / cpb, column i % cpb); a thread's
-  // elements all lie in column tid % cpb (MED_BT is a multiple of cpb).
-  // Eight loads in flight per thread, then their LDS stores.
+  // staging: element i = (segment i / CPB, column i % CPB); every element of
+  // a thread lies in column tid % CPB (MED_BT is a multiple of CPB)
   constexpr int UNR = 8;
-  const int total = nseg * cpb;
-  const int my_c = tid % cpb;
+  const int total = nseg * CPB;
+  const int my_c = tid % CPB;
   uint32_t lmin = 0xFFFFFFFFu, lmax = 0u;
   bool nan_seen = false;
   for (int base = 0; base < total; base += MED_BT * UNR) {
@@ -365,7 +368,7 @@ __global__ __launch_bounds__(MED_BT) void k_median(MedArgs a) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       const int i = base + u * MED_BT + tid;
-      v[u] = (i < total && my_c < ncol) ? src[(int64_t)(i / cpb) * a.cols + my_c] : 0.f;
+      v[u] = (i < total && my_c < ncol) ? src[(int64_t)(i / CPB) * a.cols + my_c] : 0.f;
     }
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
@@ -373,7 +376,7 @@ __global__ __launch_bounds__(MED_BT) void k_median(MedArgs a) {
       if (i < total) {
         const uint32_t k = med_key(v[u]);
         nan_seen |= v[u] != v[u];
-        tile[my_c * stride + i / cpb] = k;
+        tile[my_c * stride + i / CPB] = k;
         lmin = k < lmin ? k : lmin;
         lmax = k > lmax ? k : lmax;
       }
@@ -393,10 +396,11 @@ __global__ __launch_bounds__(MED_BT) void k_median(MedArgs a) {
   const uint32_t* keys = tile + col * stride;
   uint32_t* h = hist + col * MED_HSTRIDE;
   uint32_t prefix = 0u, rank = r0;  // prefix: bits >= hi of the selected k - kmin
-  // every column of the block runs the same number of passes (the block's
-  // largest), so the barriers are uniform; a column past its own bits is done
+  // every column of the block runs the block's largest pass count, so the
+  // barriers are uniform; a column past its own bits idles
   int hi_max = 0;
-  for (int c = 0; c < cpb; ++c) {
+#pragma unroll
+  for (int c = 0; c < CPB; ++c) {
     const uint32_t rc = kmax_s[c] - kmin_s[c];
     const int nb = rc ? 32 - __builtin_clz(rc) : 0;
     hi_max = nb > hi_max ? nb : hi_max;
@@ -406,37 +410,50 @@ __global__ __launch_bounds__(MED_BT) void k_median(MedArgs a) {
     const int lo = hi - MED_BITS > 0 ? hi - MED_BITS : 0;
     const int w = hi - lo;
     const bool active = w > 0;
-    const int nb = 1 << w;
-    const int per = (MED_BUCKETS + tpc - 1) / tpc;
-    for (int b = lt; b < MED_HSTRIDE; b += tpc) h[b] = 0u;
+    for (int b = lt; b < MED_HSTRIDE; b += TPC) h[b] = 0u;
     __syncthreads();
     if (active) {
-      const uint32_t dm = (uint32_t)nb - 1u;
-      for (int i = lt; i < nseg; i += tpc) {
+      const uint32_t dm = (1u << w) - 1u;
+      for (int i = lt; i < nseg; i += TPC) {
         const uint32_t r = keys[i] - kmin;
         if ((uint32_t)((uint64_t)r >> hi) == prefix)
           atomicAdd(&h[med_slot((int)((r >> lo) & dm))], 1u);
       }
     }
     __syncthreads();
+    uint32_t c_own[PER];
     uint32_t sum = 0u;
-    for (int b = lt * per; b < (lt + 1) * per && b < MED_BUCKETS; ++b) sum += h[med_slot(b)];
-    part[tid] = sum;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      c_own[q] = h[med_slot(lt * PER + q)];
+      sum += c_own[q];
+    }
+    // exclusive prefix of the column's per-thread sums: log-step scan over
+    // the column's lanes of each wave, then the totals of earlier waves
+    constexpr int CW = TPC < 64 ? TPC : 64;  // lanes of the column in one wave
+    const int wl = tid & 63;
+    uint32_t incl = sum;
+#pragma unroll
+    for (int d = 1; d < CW; d <<= 1) {
+      const uint32_t v = __shfl_up(incl, d, CW);
+      if ((wl % CW) >= d) incl += v;
+    }
+    if (TPC > 64 && wl == 63) part[tid >> 6] = incl;  // a column spans whole waves
     __syncthreads();
-    if (active) {
-      uint32_t before = 0u;
-      for (int j = 0; j < lt; ++j) before += part[col * tpc + j];
-      if (rank >= before && rank < before + sum) {
-        uint32_t cum = before;
-        for (int b = lt * per; b < (lt + 1) * per && b < MED_BUCKETS; ++b) {
-          const uint32_t c = h[med_slot(b)];
-          if (rank < cum + c) {
-            sel_bucket[col] = (uint32_t)b;
-            sel_rank[col] = rank - cum;
-            break;
-          }
-          cum += c;
+    uint32_t before = incl - sum;
+    if constexpr (TPC > 64)
+      for (int wv = (col * TPC) >> 6; wv < (tid >> 6); ++wv) before += part[wv];
+    if (active && rank >= before && rank < before + sum) {
+      uint32_t cum = before;
+      bool found = false;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (!found && rank < cum + c_own[q]) {
+          sel_bucket[col] = (uint32_t)(lt * PER + q);
+          sel_rank[col] = rank - cum;
+          found = true;
         }
+        cum += c_own[q];
       }
     }
     __syncthreads();
@@ -445,14 +462,14 @@ __global__ __launch_bounds__(MED_BT) void k_median(MedArgs a) {
       rank = sel_rank[col];
       hi = lo;
     }
-    __syncthreads();  // sel_* read by every thread before the next pass writes them
+    __syncthreads();  // sel_* and part read by every thread before the next pass writes them
   }
   const uint32_t k0 = kmin + prefix;
   const bool even = (nseg & 1) == 0;
   uint32_t k1 = k0;
   if (even) {
     uint32_t c = 0u, mn = 0xFFFFFFFFu;
-    for (int i = lt; i < nseg; i += tpc) {
+    for (int i = lt; i < nseg; i += TPC) {
       const uint32_t k = keys[i];
       if (k <= k0) ++c;
       else mn = k < mn ? k : mn;
@@ -1103,16 +1120,23 @@ int msig_median_frames(const float* frames, int64_t rows, int32_t nseg, int32_t 
   if (channels != 1 && channels != 2) return fail(MSIG_EINVAL, "channels must be 1 or 2");
   if (nseg > MSIG_MEDIAN_MAX_NSEG)
     return fail(MSIG_ENOTSUP, "nseg %d above the staged median's %d", nseg, MSIG_MEDIAN_MAX_NSEG);
-  int cpb = MED_MAX_CPB;
+  int cpb = MED_MAX_CPB;  // columns per block: the most that fit the LDS budget
   while (cpb > 1 && med_lds_bytes(cpb, med_stride(nseg, cpb)) > (size_t)MED_LDS_BUDGET) cpb /= 2;
   const int stride = med_stride(nseg, cpb);
   const size_t lds = med_lds_bytes(cpb, stride);
   const int ncb = (cols + cpb - 1) / cpb;
   if (rows * (int64_t)ncb > INT32_MAX) return fail(MSIG_EINVAL, "too many columns");
-  MedArgs m{frames, out, rows, nseg, cols, cpb, ncb, channels, apply_factor ? 1 : 0, nfft,
-            stride, (float)scale};
-  hipLaunchKernelGGL(k_median, dim3((unsigned)(rows * ncb)), dim3(MED_BT), lds,
-                     static_cast<hipStream_t>(stream), m);
+  MedArgs m{frames, out, rows, nseg, cols, ncb, channels, apply_factor ? 1 : 0, nfft, stride,
+            (float)scale};
+  const dim3 grid((unsigned)(rows * ncb)), block(MED_BT);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (cpb) {
+    case 16: hipLaunchKernelGGL(k_median<16>, grid, block, lds, s, m); break;
+    case 8: hipLaunchKernelGGL(k_median<8>, grid, block, lds, s, m); break;
+    case 4: hipLaunchKernelGGL(k_median<4>, grid, block, lds, s, m); break;
+    case 2: hipLaunchKernelGGL(k_median<2>, grid, block, lds, s, m); break;
+    default: hipLaunchKernelGGL(k_median<1>, grid, block, lds, s, m); break;
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "median kernel launch");
   return 0;
