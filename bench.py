@@ -8,7 +8,10 @@ Inputs are generated on the device from a seed (synthetic) and are resident
 in HBM before the timed region; three input buffers are rotated so the 256 MB
 last-level cache cannot serve the 256 MiB input.
 
-One step = one ``mlx_signal_amd.welch`` call on a device buffer.  Prints ONE
+One step = one ``mlx_signal_amd.welch`` call on a device buffer.  Every timed
+region follows at least W warm-up steps and at least 80 ms of back-to-back
+work, so it measures the sustained rate (an idle GPU runs its first ~30-40 ms
+of work at lower clocks).  Prints ONE
 JSON line (value = samples/s over the whole job).  Secondary forms (csd,
 coherence at the same shape, stft / spectrogram 16 x 2^20) are reported under
 "secondary".  The CPU baseline is the reference's dispatch='scipy' route
@@ -65,16 +68,38 @@ def make_inputs(torch, rows, n, seed, dev, with_y=True):
     return x.contiguous(), y.contiguous()
 
 
+WARM_MS = 80.0  # back-to-back work before any timed region
+
+
+def warm_up(torch, fn, bufs, warmup, min_ms=WARM_MS):
+    """At least `warmup` steps and at least `min_ms` of synchronized back-to-back
+    work.  An idle MI355X runs its first ~30-40 ms of sustained work at lower
+    clocks (the fused welch kernel takes ~105 us cold, 86 us sustained), so a
+    short warm-up times the ramp rather than the sustained rate.  Returns the
+    number of warm-up steps run."""
+    n = 0
+    for i in range(warmup):
+        fn(bufs[n % len(bufs)])
+        n += 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while (time.perf_counter() - t0) * 1e3 < min_ms:
+        for _ in range(8):
+            fn(bufs[n % len(bufs)])
+            n += 1
+        torch.cuda.synchronize()
+    return n
+
+
 def time_steps(torch, fn, bufs, steps, warmup, kernel_timing=False):
     """Wall clock over K steps (synchronized both sides) plus per-step HIP event
-    durations on the current stream (the stream the kernels launch on).  With
-    kernel_timing, the library also records HIP events on that stream around
-    each step's fused transform kernel; returns its mean duration too."""
+    durations on the current stream (the stream the kernels launch on), after
+    warm_up().  With kernel_timing, the library also records HIP events on that
+    stream around each step's fused transform kernel; returns its mean
+    duration too."""
     from mlx_signal_amd import _kernels
 
-    for i in range(warmup):
-        fn(bufs[i % len(bufs)])
-    torch.cuda.synchronize()
+    warm_up(torch, fn, bufs, warmup)
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(steps)]
     if kernel_timing:
@@ -285,8 +310,7 @@ def bench_ragged(torch, msig, dev, reps=3):
                 msig.csd(r.x, r.y, **kw)
 
     def timed(fn):
-        fn()
-        torch.cuda.synchronize()
+        warm_up(torch, lambda _: fn(), [None], 1)
         ts = []
         for _ in range(reps):
             e0 = torch.cuda.Event(enable_timing=True)
@@ -442,8 +466,8 @@ def bench_serving(torch, msig, dev):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-ref", action="store_true",
@@ -571,6 +595,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_min_ms": WARM_MS,
         "ms_per_step": ms_wall,
         "ms_per_step_event": ms_ev,
         "higher_is_better": True,

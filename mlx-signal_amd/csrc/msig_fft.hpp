@@ -241,6 +241,53 @@ struct Plan {
     const int vt = t + TPF * u;
     return (j * (vt % NS) * (N / (NS * R))) % N;
   }
+
+  // Shared LDS twiddle table (one per block).  Pass p's twiddle for
+  // register u R + j depends on vt % NS, vt = t + TPF u.  When NS <= TPF
+  // (NS divides TPF) that is t % NS for every u, so the pass can store R - 1
+  // slots of NS columns ("narrow"); otherwise it stores (E / R)(R - 1) slots
+  // of TPF columns.  Slot-major either way: the lanes of a wave read
+  // consecutive (or broadcast) entries of one slot.  Narrow passes are used
+  // only when the all-wide table would exceed 16 KB (N >= 2048 here): below
+  // that the table costs no occupancy, and with one base address for a whole
+  // narrow pass the compiler serialises its reads (one LDS round trip per
+  // pair), measured 4-6% slower at N = 1024.
+  static constexpr bool TWL_DEDUP = TWN * TPF * 8 > 16384;
+  static constexpr bool twl_narrow(int p) { return TWL_DEDUP && ns(p) <= TPF; }
+  static constexpr int twl_cols(int p) { return twl_narrow(p) ? ns(p) : TPF; }
+  static constexpr int twl_slots(int p) {
+    return twl_narrow(p) ? radix(p) - 1 : (E / radix(p)) * (radix(p) - 1);
+  }
+  static constexpr int twl_off(int p) {
+    int o = 0;
+    for (int i = 1; i < p; ++i) o += twl_slots(i) * twl_cols(i);
+    return o;
+  }
+  static constexpr int TWL = twl_off(NP) > 0 ? twl_off(NP) : 1;
+  // column of thread t in pass P's table
+  template <int P>
+  static MSIG_HD int twl_col(int t) {
+    return twl_narrow(P) ? (t & (ns(P) - 1)) : t;
+  }
+  // slot of register u R + j (j >= 1) in pass P's table
+  static constexpr int twl_slot(int p, int u, int j) {
+    return twl_narrow(p) ? j - 1 : u * (radix(p) - 1) + (j - 1);
+  }
+  // twiddle exponent stored at table entry i (0 <= i < TWL)
+  template <int P = 1>
+  static MSIG_HD int twl_exp(int i) {
+    if constexpr (P >= NP) {
+      return 0;
+    } else {
+      constexpr int R = radix(P), NS = ns(P), C = twl_cols(P), SZ = twl_slots(P) * C;
+      if (i >= SZ) return twl_exp<P + 1>(i - SZ);
+      const int slot = i / C, col = i % C;
+      const int u = twl_narrow(P) ? 0 : slot / (R - 1);
+      const int j = twl_narrow(P) ? slot + 1 : slot % (R - 1) + 1;
+      const int vt = col + TPF * u;
+      return (j * (vt % NS) * (N / (NS * R))) % N;
+    }
+  }
 };
 
 // Load the per-thread twiddle registers from an N-entry table of
@@ -263,21 +310,25 @@ MSIG_HD void load_twiddles(int t, TablePtr table, cf* tw) {
   }
 }
 
-// Pass P compute step on registers already read for that pass.  Twiddle
-// slot s of this thread is tw[s * TS]: TS = 1 for a per-thread register
-// array, TS = TPF for the slot-major LDS table (one column per thread
-// position, so every read is one ds_read_b64 at a constant offset).
-template <int N, int E, int P, int TS = 1, int S = 0>
-MSIG_HD void pass_compute(cf* v, const cf* tw) {
+// Pass P compute step on registers already read for that pass.  TWL = false:
+// tw is this thread's register array from load_twiddles.  TWL = true: tw is
+// the block's shared table (Plan::twl_*), read at this thread's column, so
+// every read is one ds_read_b64 at a constant offset from a per-pass base.
+template <int N, int E, int P, bool TWL = false, int S = 0>
+MSIG_HD void pass_compute(cf* v, const cf* tw, int t = 0) {
   using PL = Plan<N, E, S>;
   constexpr int R = PL::radix(P);
+  const cf* twp = tw;
+  if constexpr (TWL && P > 0) twp = tw + PL::twl_off(P) + PL::template twl_col<P>(t);
 #pragma unroll
   for (int u = 0; u < E / R; ++u) {
     if constexpr (P > 0) {
       constexpr int off = PL::tw_off(P);
 #pragma unroll
-      for (int j = 1; j < R; ++j)
-        v[u * R + j] = cmul(v[u * R + j], tw[(off + u * (R - 1) + (j - 1)) * TS]);
+      for (int j = 1; j < R; ++j) {
+        const cf w = TWL ? twp[PL::twl_slot(P, u, j) * PL::twl_cols(P)] : twp[off + u * (R - 1) + (j - 1)];
+        v[u * R + j] = cmul(v[u * R + j], w);
+      }
     }
     Dft<R>::run(v + u * R);
   }

@@ -110,34 +110,70 @@ __device__ __forceinline__ void group_sum2(float& a, float& b, float* red) {
   }
 }
 
-// Where a kernel keeps its Stockham twiddles: in an LDS table (slot-major,
-// TWN x TPF) when that table is small, else in registers.  The LDS table
-// frees ~2*TWN VGPRs per thread, which buys occupancy.
-template <int N, int E, int S = 0>
+// Where a kernel keeps its Stockham twiddles: in the block's shared LDS
+// table (Plan::twl_*, deduplicated across threads that share a twiddle) when
+// the kernel wants it (WANT) and it fits in 32 KB, else in registers.  The
+// table frees ~2*TWN VGPRs per thread, which buys occupancy.
+template <int N, int E, int S = 0, bool WANT = true>
 struct TwiddleHome {
   using PL = Plan<N, E, S>;
-  static constexpr bool LDS = (PL::NP > 1) && (PL::TWN * PL::TPF * 8 <= 32768);
-  static constexpr int LDS_ELEMS = LDS ? PL::TWN * PL::TPF : 1;
+  static constexpr bool LDS = WANT && (PL::NP > 1) && (PL::TWL * 8 <= 32768);
+  static constexpr int LDS_ELEMS = LDS ? PL::TWL : 1;
   static constexpr int REGS = LDS ? 1 : PL::TWN;
-  static constexpr int TS = LDS ? PL::TPF : 1;
 };
 
-// Fill the slot-major LDS twiddle table from the global FP64-derived table.
+// Which kernels want the LDS table.  Up to N = 2048 (tables <= 16 KB) all
+// do.  At N = 4096 the 32 KB table pays only where the VGPRs it frees let
+// the kernel run two waves per SIMD instead of one, which is the two-signal
+// hop == N/2 mean (measured 358 -> 273 us on 64 x 2^20 csd); the others
+// stay at the same occupancy either way and only pay for the extra LDS
+// reads (welch 125 -> 136 us, coherence 363 -> 413 us), so they keep
+// register twiddles.
+template <int N, int KIND, bool HALF>
+constexpr bool mean_twl() {
+  return N < 4096 || (KIND == K_CSD && HALF);
+}
+template <int N>
+constexpr bool frames_twl() {
+  return N < 4096;
+}
+
+// Fill the shared LDS twiddle table from the global FP64-derived table.  All
+// of a thread's loads are issued before any LDS write, so the fill costs one
+// L2 round trip per block rather than one per entry.  An all-wide table is
+// column t = thread t's load_twiddles registers; a deduplicated one is
+// filled entry by entry from Plan::twl_exp.
 template <int N, int E, int S = 0>
 __device__ __forceinline__ void fill_twiddle_lds(cf* twl, const cf* table) {
   using PL = Plan<N, E, S>;
-  for (int t = threadIdx.x; t < PL::TPF; t += BT) {
-    cf tmp[PL::TWN];
-    load_twiddles<N, E, 1, const MSIG_GLOBAL cf*, S>(t, as_global(table), tmp);
+  const MSIG_GLOBAL cf* g = as_global(table);
+  if constexpr (!PL::TWL_DEDUP) {
+    for (int t = threadIdx.x; t < PL::TPF; t += BT) {
+      cf tmp[PL::TWN];
+      load_twiddles<N, E, 1, const MSIG_GLOBAL cf*, S>(t, g, tmp);
 #pragma unroll
-    for (int s = 0; s < PL::TWN; ++s) twl[s * PL::TPF + t] = tmp[s];
+      for (int s = 0; s < PL::TWN; ++s) twl[s * PL::TPF + t] = tmp[s];
+    }
+  } else {
+    constexpr int ITER = (PL::TWL + BT - 1) / BT;
+    cf tmp[ITER];
+#pragma unroll
+    for (int k = 0; k < ITER; ++k) {
+      const int i = (int)threadIdx.x + k * BT;
+      if (i < PL::TWL) tmp[k] = g[PL::twl_exp(i)];
+    }
+#pragma unroll
+    for (int k = 0; k < ITER; ++k) {
+      const int i = (int)threadIdx.x + k * BT;
+      if (i < PL::TWL) twl[i] = tmp[k];
+    }
   }
 }
 
 // Forward FFT of the group's N points held E per thread (register e holds
-// sample t + TPF e); tw points at this thread's twiddle slot 0 (stride TS
-// between slots); radix schedule S (Plan).
-template <int N, int E, int TS, int P = 1, int S = 0>
+// sample t + TPF e); tw is this thread's twiddle registers (TWL = false) or
+// the block's shared LDS table (TWL = true); radix schedule S (Plan).
+template <int N, int E, bool TWL, int P = 1, int S = 0>
 __device__ __forceinline__ void fft_passes(int t, cf* v, const cf* tw, cf* lds) {
   using PL = Plan<N, E, S>;
   if constexpr (P == 1) {
@@ -148,15 +184,15 @@ __device__ __forceinline__ void fft_passes(int t, cf* v, const cf* tw, cf* lds) 
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = tmp[e];
     }
-    pass_compute<N, E, 0, TS, S>(v, tw);
+    pass_compute<N, E, 0, TWL, S>(v, tw, t);
   }
   if constexpr (P < PL::NP) {
     pass_store<N, E, P - 1, S>(t, v, lds);
     group_sync<PL::TPF>();
     pass_load<N, E, P, S>(t, v, lds);
     if constexpr (PL::TPF > 64) __syncthreads();  // reads done before next stores
-    pass_compute<N, E, P, TS, S>(v, tw);
-    fft_passes<N, E, TS, P + 1, S>(t, v, tw, lds);
+    pass_compute<N, E, P, TWL, S>(v, tw, t);
+    fft_passes<N, E, TWL, P + 1, S>(t, v, tw, lds);
   }
 }
 
@@ -204,11 +240,11 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
   cf* lds = lds_all + g * PL::LDS;
   const int unit0 = (bir * GPB + g) * a.run;
 
-  using TH = TwiddleHome<N, E, S>;
+  using TH = TwiddleHome<N, E, S, mean_twl<N, KIND, HALF>()>;
   cf twr[TH::REGS];
   const cf* tw;
   if constexpr (TH::LDS) {
-    tw = twl + t;
+    tw = twl;
   } else {
     load_twiddles<N, E, 1, const MSIG_GLOBAL cf*, S>(t, as_global(a.twt), twr);
     tw = twr;
@@ -238,17 +274,17 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
   auto unit_body = [&](cf (&zx)[E], cf (&zy)[E], bool active) {
     if constexpr (KIND == K_PSD) {
       prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS, 1, S>(t, zx, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S>(t, zx, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) acc[0][e] = fmaf(zx[e].im, zx[e].im, fmaf(zx[e].re, zx[e].re, acc[0][e]));
       }
     } else {
       prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS, 1, S>(t, zx, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S>(t, zx, tw, lds);
       group_sync<TPF>();  // last-pass reads of zx done before zy's stores
       prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS, 1, S>(t, zy, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S>(t, zy, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -317,7 +353,7 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
     if constexpr (KIND == K_PSD) {
       if (more) sx.fetch(unit + 1, false, a);
       prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS, 1, S>(t, zx, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S>(t, zx, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) acc[0][e] = fmaf(zx[e].im, zx[e].im, fmaf(zx[e].re, zx[e].re, acc[0][e]));
@@ -330,10 +366,10 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
         sy.fetch(unit + 1, false, a);
       }
       prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS, 1, S>(t, zx, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S>(t, zx, tw, lds);
       group_sync<TPF>();  // last-pass reads of zx done before zy's stores
       prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::TS, 1, S>(t, zy, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S>(t, zy, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -378,12 +414,19 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
 }
 
 // Waves per SIMD each kernel is compiled for.  Single-signal kinds fit three
-// waves without spills; two-signal kinds hold both spectra and their
-// prefetch buffers, and run with two (CSD) or as many as fit (coherence).
-template <int N, int KIND>
+// waves without spills up to N = 1024 (and the hop == N/2 form at 2048,
+// whose deduplicated 16 KB twiddle table leaves LDS for three blocks) and
+// two above; two-signal kinds hold both spectra and their prefetch buffers
+// and run with two (CSD; at N = 4096 only the hop == N/2 form, whose
+// twiddles move to LDS) or one (coherence).
+template <int N, int KIND, bool HALF>
 constexpr int mean_waves() {
-  return N >= 4096 ? (KIND == K_PSD ? 2 : 1)
-                   : N >= 2048 ? (KIND == K_PSD ? 2 : 1) : (KIND == K_PSD ? 3 : (KIND == K_CSD ? 2 : 1));
+  if constexpr (N >= 2048) {
+    if constexpr (KIND == K_PSD) return (N == 2048 && HALF) ? 3 : 2;
+    if constexpr (KIND == K_CSD) return (N < 4096 || HALF) ? 2 : 1;
+    return 1;
+  }
+  return KIND == K_PSD ? 3 : (KIND == K_CSD ? 2 : 1);
 }
 template <int N, int OK>
 constexpr int frames_waves() {
@@ -393,12 +436,13 @@ constexpr int frames_waves() {
 
 // HALF selects the hop == N/2 register-reuse stream (real input, nperseg == N).
 template <int N, int E, int KIND, bool CPLX, bool HALF>
-__global__ __launch_bounds__(BT, (mean_waves<N, KIND>())) void k_mean(const KArgs a) {
+__global__ __launch_bounds__(BT, (mean_waves<N, KIND, HALF>())) void k_mean(const KArgs a) {
   using PL = Plan<N, E>;
+  using TH = TwiddleHome<N, E, 0, mean_twl<N, KIND, HALF>()>;
   __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
   __shared__ float red[2 * (BT / 64)];
-  __shared__ cf twl[TwiddleHome<N, E>::LDS_ELEMS];
-  if constexpr (TwiddleHome<N, E>::LDS) {
+  __shared__ cf twl[TH::LDS_ELEMS];
+  if constexpr (TH::LDS) {
     fill_twiddle_lds<N, E>(twl, a.twt);
     __syncthreads();
   }
@@ -439,11 +483,11 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
   cf* lds = lds_all + g * PL::LDS;
   const int unit0 = (bir * GPB + g) * a.run;
 
-  using TH = TwiddleHome<N, E>;
+  using TH = TwiddleHome<N, E, 0, frames_twl<N>()>;
   cf twr[TH::REGS];
   const cf* tw;
   if constexpr (TH::LDS) {
-    tw = twl + t;
+    tw = twl;
   } else {
     load_twiddles<N, E>(t, as_global(a.twt), twr);
     tw = twr;
@@ -479,13 +523,13 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
       if constexpr (PAIR) sy.fetch(unit + 1, false, a);
     }
     prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-    fft_passes<N, E, TH::TS>(t, zx, tw, lds);
+    fft_passes<N, E, TH::LDS>(t, zx, tw, lds);
     if constexpr (CPLX) {
       // one segment per unit; bins come out in registers, no separation
       if constexpr (PAIR) {
         group_sync<TPF>();
         prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
-        fft_passes<N, E, TH::TS>(t, zy, tw, lds);
+        fft_passes<N, E, TH::LDS>(t, zy, tw, lds);
       }
       if (active) {
         const int64_t fr = row * a.nseg + unit;
@@ -514,7 +558,7 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
       cf y0[E], y1[E];
       if constexpr (PAIR) {
         prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
-        fft_passes<N, E, TH::TS>(t, zy, tw, lds);
+        fft_passes<N, E, TH::LDS>(t, zy, tw, lds);
         separate<N, E>(t, zy, lds, y0, y1);
       }
       if (active) {
@@ -558,10 +602,11 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
 template <int N, int E, int OK, bool CPLX, bool HALF>
 __global__ __launch_bounds__(BT, (frames_waves<N, OK>())) void k_frames(const KArgs a) {
   using PL = Plan<N, E>;
+  using TH = TwiddleHome<N, E, 0, frames_twl<N>()>;
   __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
   __shared__ float red[2 * (BT / 64)];
-  __shared__ cf twl[TwiddleHome<N, E>::LDS_ELEMS];
-  if constexpr (TwiddleHome<N, E>::LDS) {
+  __shared__ cf twl[TH::LDS_ELEMS];
+  if constexpr (TH::LDS) {
     fill_twiddle_lds<N, E>(twl, a.twt);
     __syncthreads();
   }
@@ -589,11 +634,11 @@ __device__ __forceinline__ void istft_body(const KArgs& a, int64_t row, int bir,
   cf* lds = lds_all + g * PL::LDS;
   const int unit0 = (bir * GPB + g) * a.run;
 
-  using TH = TwiddleHome<N, E>;
+  using TH = TwiddleHome<N, E, 0, frames_twl<N>()>;
   cf twr[TH::REGS];
   const cf* tw;
   if constexpr (TH::LDS) {
-    tw = twl + t;
+    tw = twl;
   } else {
     load_twiddles<N, E>(t, as_global(a.twt), twr);
     tw = twr;
@@ -630,7 +675,7 @@ __device__ __forceinline__ void istft_body(const KArgs& a, int64_t row, int bir,
       // conj(X_s + i X_t)
       z[e] = cf{xs.re - xt.im, -(xs.im + xt.re)};
     }
-    fft_passes<N, E, TH::TS>(t, z, tw, lds);
+    fft_passes<N, E, TH::LDS>(t, z, tw, lds);
     // sample n = bin(t, e): x_s = Re F / N, x_t = -Im F / N (1/N folded in w)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -645,9 +690,10 @@ __device__ __forceinline__ void istft_body(const KArgs& a, int64_t row, int bir,
 template <int N, int E>
 __global__ __launch_bounds__(BT) void k_istft(const KArgs a) {
   using PL = Plan<N, E>;
+  using TH = TwiddleHome<N, E, 0, frames_twl<N>()>;
   __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
-  __shared__ cf twl[TwiddleHome<N, E>::LDS_ELEMS];
-  if constexpr (TwiddleHome<N, E>::LDS) {
+  __shared__ cf twl[TH::LDS_ELEMS];
+  if constexpr (TH::LDS) {
     fill_twiddle_lds<N, E>(twl, a.twt);
     __syncthreads();
   }
@@ -667,25 +713,26 @@ struct WorkItem {
 // is the same alone and in the batch.
 // The batch kernels run at the standalone kernels' occupancy (measured
 // faster on ragged batches than one wave fewer).
-template <int N, int KIND>
+template <int N, int KIND, bool HALF>
 constexpr int batch_waves() {
-  return mean_waves<N, KIND>();
+  return mean_waves<N, KIND, HALF>();
 }
 
 template <int N, int E, int KIND, bool CPLX, bool HALF, int S>
-__global__ __launch_bounds__(BT, (batch_waves<N, KIND>())) void k_mean_batch(
+__global__ __launch_bounds__(BT, (batch_waves<N, KIND, HALF>())) void k_mean_batch(
     const KArgs* __restrict__ jobs, const WorkItem* __restrict__ queue, int32_t nwork,
     float* ws_base) {
   using PL = Plan<N, E>;
+  using TH = TwiddleHome<N, E, S, mean_twl<N, KIND, HALF>()>;
   __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
   __shared__ float red[2 * (BT / 64)];
-  __shared__ cf twl[TwiddleHome<N, E, S>::LDS_ELEMS];
+  __shared__ cf twl[TH::LDS_ELEMS];
   bool filled = false;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const WorkItem item = queue[w];
     const KArgs& a = jobs[item.job];
     float* ws = ws_base + a.ws_off;
-    if constexpr (TwiddleHome<N, E, S>::LDS) {
+    if constexpr (TH::LDS) {
       if (!filled) {  // every job of a class has the same N: fill once
         fill_twiddle_lds<N, E, S>(twl, a.twt);
         filled = true;

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstring>
 #include <random>
 #include <vector>
 
@@ -37,9 +38,9 @@ static std::vector<cd> dft(const std::vector<cd>& z) {
   return out;
 }
 
-template <int N, int E, int P, int S = 0>
+template <int N, int E, int P, int S = 0, bool TWL = false>
 static bool run_pass(std::vector<std::vector<cf>>& v, const std::vector<std::vector<cf>>& tw,
-                     std::vector<cf>& lds, std::vector<int>& wcount) {
+                     const std::vector<cf>& twl, std::vector<cf>& lds, std::vector<int>& wcount) {
   using PL = msig::Plan<N, E, S>;
   constexpr int TPF = PL::TPF;
   if constexpr (P < PL::NP) {
@@ -63,20 +64,25 @@ static bool run_pass(std::vector<std::vector<cf>>& v, const std::vector<std::vec
       }
       msig::pass_load<N, E, P, S>(t, v[t].data(), lds.data());
     }
-    for (int t = 0; t < TPF; ++t) msig::pass_compute<N, E, P, 1, S>(v[t].data(), tw[t].data());
-    return run_pass<N, E, P + 1, S>(v, tw, lds, wcount);
+    for (int t = 0; t < TPF; ++t)
+      msig::pass_compute<N, E, P, TWL, S>(v[t].data(), TWL ? twl.data() : tw[t].data(), t);
+    return run_pass<N, E, P + 1, S, TWL>(v, tw, twl, lds, wcount);
   }
   return true;
 }
 
 // FFT of z (N complex points) exactly as a group of TPF device threads does it
 // (radix schedule S; register e of a thread starts with sample t + TPF e and
-// is renamed to pass 0's order as fft_passes does).
-template <int N, int E, int S = 0>
+// is renamed to pass 0's order as fft_passes does).  TWL = true reads the
+// twiddles from the block-shared table filled as fill_twiddle_lds does
+// (entry i = table[Plan::twl_exp(i)]) instead of per-thread registers.
+template <int N, int E, int S = 0, bool TWL = false>
 static bool emu_fft(const std::vector<cf>& table, const std::vector<cf>& z, std::vector<cf>& Z) {
   using PL = msig::Plan<N, E, S>;
   constexpr int TPF = PL::TPF;
   std::vector<std::vector<cf>> v(TPF, std::vector<cf>(E)), tw(TPF, std::vector<cf>(PL::TWN));
+  std::vector<cf> twl(PL::TWL);
+  for (int i = 0; i < PL::TWL; ++i) twl[i] = table[PL::twl_exp(i)];
   for (int t = 0; t < TPF; ++t) {
     msig::load_twiddles<N, E, 1, const cf*, S>(t, table.data(), tw[t].data());
     for (int e = 0; e < E; ++e) {
@@ -87,11 +93,11 @@ static bool emu_fft(const std::vector<cf>& table, const std::vector<cf>& z, std:
       }
       v[t][e] = z[t + TPF * src];
     }
-    msig::pass_compute<N, E, 0, 1, S>(v[t].data(), tw[t].data());
+    msig::pass_compute<N, E, 0, TWL, S>(v[t].data(), TWL ? twl.data() : tw[t].data(), t);
   }
   std::vector<cf> lds(PL::LDS);
   std::vector<int> wcount(PL::LDS);
-  if (!run_pass<N, E, 1, S>(v, tw, lds, wcount)) return false;
+  if (!run_pass<N, E, 1, S, TWL>(v, tw, twl, lds, wcount)) return false;
   Z.assign(N, cf{NAN, NAN});
   std::vector<int> seen(N, 0);
   for (int t = 0; t < TPF; ++t)
@@ -398,6 +404,59 @@ static int check_schedule(std::mt19937_64& rng) {
   return ok ? 0 : 1;
 }
 
+// The block-shared LDS twiddle table: every (pass, thread, register) read
+// lands on an entry holding that register's twiddle exponent, and the FFT
+// through the table is bit-identical to the per-thread register form.
+template <int N, int E, int S, int P = 1>
+static bool twl_index_ok() {
+  using PL = msig::Plan<N, E, S>;
+  if constexpr (P >= PL::NP) {
+    return true;
+  } else {
+    constexpr int R = PL::radix(P);
+    for (int t = 0; t < PL::TPF; ++t)
+      for (int u = 0; u < E / R; ++u)
+        for (int j = 1; j < R; ++j) {
+          const int i = PL::twl_off(P) + PL::twl_slot(P, u, j) * PL::twl_cols(P) + PL::template twl_col<P>(t);
+          if (i < 0 || i >= PL::TWL || PL::twl_exp(i) != PL::template tw_exp<P>(t, u * R + j)) {
+            std::printf("N=%d S=%d pass %d t=%d u=%d j=%d: table entry %d wrong\n", N, S, P, t, u, j, i);
+            return false;
+          }
+        }
+    return twl_index_ok<N, E, S, P + 1>();
+  }
+}
+
+template <int N, int S = 0>
+static int check_shared_table(std::mt19937_64& rng) {
+  constexpr int E = N < 16 ? N : 16;
+  using PL = msig::Plan<N, E, S>;
+  std::vector<cf> table(N);
+  for (int i = 0; i < N; ++i) {
+    const double a = -2.0 * M_PI * i / N;
+    table[i] = cf{(float)std::cos(a), (float)std::sin(a)};
+  }
+  std::normal_distribution<double> nd;
+  std::vector<cf> z(N), Zr, Zl;
+  for (int i = 0; i < N; ++i) z[i] = cf{(float)nd(rng), (float)nd(rng)};
+  bool ok = twl_index_ok<N, E, S>();
+  if (!PL::TWL_DEDUP) {
+    // an all-wide table is column t = thread t's load_twiddles registers
+    std::vector<cf> regs(PL::TWN);
+    for (int t = 0; ok && t < PL::TPF; ++t) {
+      msig::load_twiddles<N, E, 1, const cf*, S>(t, table.data(), regs.data());
+      for (int s = 0; ok && s < PL::TWN; ++s)
+        ok = std::memcmp(&regs[s], &table[PL::twl_exp(s * PL::TPF + t)], sizeof(cf)) == 0;
+    }
+  }
+  ok = ok && emu_fft<N, E, S, false>(table, z, Zr) && emu_fft<N, E, S, true>(table, z, Zl);
+  for (int k = 0; ok && k < N; ++k)
+    ok = std::memcmp(&Zr[k], &Zl[k], sizeof(cf)) == 0;
+  std::printf("shared twiddle table N=%5d S=%d entries=%d (%d bytes, per-thread form %d) %s\n", N, S,
+              PL::TWL, PL::TWL * 8, PL::TWN * PL::TPF * 8, ok ? "ok" : "FAIL");
+  return ok ? 0 : 1;
+}
+
 int main() {
   std::mt19937_64 rng(1234);
   int bad = 0;
@@ -410,6 +469,14 @@ int main() {
   bad += check_size<1024>(rng);
   bad += check_size<2048>(rng);
   bad += check_size<4096>(rng);
+  bad += check_shared_table<128>(rng);
+  bad += check_shared_table<256>(rng);
+  bad += check_shared_table<512>(rng);
+  bad += check_shared_table<1024>(rng);
+  bad += check_shared_table<2048>(rng);
+  bad += check_shared_table<4096>(rng);
+  bad += check_shared_table<512, 1>(rng);
+  bad += check_shared_table<2048, 1>(rng);
   // hop == N/2 register reuse: odd and even segment counts
   bad += check_stream<64, true>(rng, 64, 32, 64 + 32 * 14, 1);   // nseg 15
   bad += check_stream<64, true>(rng, 64, 32, 64 + 32 * 15, 0);   // nseg 16
