@@ -38,8 +38,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Barrier for the threads of one FFT group.  Groups that fit in a wavefront
 // only need LDS ordering within the wave; larger groups span several waves
-// and use the block barrier (every group of the block runs the same loop
-// trip count, so all threads reach it).
+// and use the block barrier, so every group of a block must run the same
+// sequence of barriers (the mean, frames and istft loops give every group
+// the same trip count and the same barrier-bearing steps per unit).
 template <int TPF>
 __device__ __forceinline__ void group_sync() {
   if constexpr (TPF <= 64) {
@@ -270,20 +271,20 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[c][e] = 0.f;
 
-  // one unit: z already taken, detrend + window + FFT + accumulate
-  auto unit_body = [&](cf (&zx)[E], cf (&zy)[E], bool active) {
+  // FFT + accumulate of a unit whose x points are detrended and windowed;
+  // prep_y() fills zy after zx's FFT (two-signal kinds), so only one
+  // signal's windowed points are live at a time.
+  auto transform_acc = [&](cf (&zx)[E], cf (&zy)[E], bool active, auto&& prep_y) {
     if constexpr (KIND == K_PSD) {
-      prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
       fft_passes<N, E, TH::LDS, 1, S>(t, zx, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) acc[0][e] = fmaf(zx[e].im, zx[e].im, fmaf(zx[e].re, zx[e].re, acc[0][e]));
       }
     } else {
-      prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
       fft_passes<N, E, TH::LDS, 1, S>(t, zx, tw, lds);
       group_sync<TPF>();  // last-pass reads of zx done before zy's stores
-      prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
+      prep_y();
       fft_passes<N, E, TH::LDS, 1, S>(t, zy, tw, lds);
       if (active) {
 #pragma unroll
@@ -308,30 +309,71 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
     group_sync<TPF>();  // reads of this unit's last pass done before next stores
   };
 
+  // one unit: z already taken, detrend + window + FFT + accumulate
+  auto unit_body = [&](cf (&zx)[E], cf (&zy)[E], bool active) {
+    prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
+    transform_acc(zx, zy, active, [&] { prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red); });
+  };
+
   if constexpr (HALF) {
     // Units [unit0, u1) of this group; those below nseg / 2 have both
     // segments and run with unconditional loads (the prefetch past the last
-    // of them re-reads that unit, in bounds and unused).  The partner-less
-    // unit of an odd segment count runs once after them.
+    // of them re-reads that unit, in bounds and unused).  Unit u is
+    // half-blocks h_2u, h_2u+1, h_2u+2, so consecutive units share a half:
+    // its detrend group sum is carried to the next unit instead of being
+    // recomputed (each unit sums its two new halves).  The partner-less unit
+    // of an odd segment count runs once after them.
     const int u1 = unit0 + a.run < a.units ? unit0 + a.run : a.units;
     const int full = (int)(a.nseg / 2);
     const int uf = u1 < full ? u1 : full;
-    if (unit0 < uf) {
+    const bool has_full = unit0 < uf;
+    const bool has_tail = full >= unit0 && full < u1;
+    const float inv = 1.0f / (float)a.nperseg;
+    float sxc = 0.f, syc = 0.f;  // group sums of the carried half (detrend)
+    if (has_full) {
       sx.fetch_full(unit0, true);
       if constexpr (KIND != K_PSD) sy.fetch_full(unit0, true);
+    }
+    if (a.detrend) {
+      if (has_full) {
+        sxc = sx.first_half_sum();
+        if constexpr (KIND != K_PSD) syc = sy.first_half_sum();
+      }
+      group_sum2<TPF>(sxc, syc, red);  // every group: the same barriers (see below)
+    }
+    // segment means of the fetched unit of one signal (detrend)
+    auto means = [&](const UnitStream<N, E, CPLX, HALF>& st, float& carry, float& mr, float& mi) {
+      mr = 0.f;
+      mi = 0.f;
+      if (a.detrend) {
+        float sb, sc;
+        st.new_half_sums(sb, sc);
+        group_sum2<TPF>(sb, sc, red);
+        st.half_means(carry, sb, sc, inv, mr, mi);
+      }
+    };
+    if (has_full) {
       for (int unit = unit0; unit < uf; ++unit) {
         const int nxt = unit + 1 < uf ? unit + 1 : uf - 1;
         cf zx[E], zy[E];
-        sx.take_full(zx);
+        float mrx, mix;
+        means(sx, sxc, mrx, mix);
+        sx.take_full_prepped(zx, w, a.detrend, mrx, mix);
         sx.fetch_full(nxt, false);
-        if constexpr (KIND != K_PSD) {
-          sy.take_full(zy);
+        if constexpr (KIND == K_PSD) {
+          transform_acc(zx, zy, true, [] {});
+        } else {
+          const UnitStream<N, E, CPLX, HALF> ycur = sy;  // this unit's raw y halves
           sy.fetch_full(nxt, false);
+          transform_acc(zx, zy, true, [&] {
+            float mry, miy;
+            means(ycur, syc, mry, miy);
+            ycur.take_full_prepped(zy, w, a.detrend, mry, miy);
+          });
         }
-        unit_body(zx, zy, true);
       }
     }
-    if (full >= unit0 && full < u1) {
+    if (has_tail) {
       cf zx[E], zy[E];
       sx.fetch(full, true, a);
       sx.take(zx);
@@ -340,6 +382,27 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
         sy.take(zy);
       }
       unit_body(zx, zy, true);
+    }
+    if constexpr (TPF > 64 && GPB > 1) {
+      // Several multi-wave groups per block (N = 2048), which synchronise with
+      // block barriers: every group of a block must run the same sequence of
+      // barriers (different sequences in the two groups corrupted results on
+      // MI355X).  Each unit above runs one detrend reduction per signal and
+      // the FFTs in the same order, so a group with fewer units than the
+      // block's first group (the last block of a row) runs inactive units on
+      // zeros up to that count.
+      const int ran = u1 > unit0 ? u1 - unit0 : 0;
+      int most = a.units - bir * GPB * a.run;
+      most = most < a.run ? most : a.run;
+      for (int k = ran; k < most; ++k) {
+        cf zx[E], zy[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          zx[e] = cf{0.f, 0.f};
+          zy[e] = cf{0.f, 0.f};
+        }
+        unit_body(zx, zy, false);
+      }
     }
   } else {
   sx.fetch(unit0, true, a);
@@ -659,29 +722,38 @@ __device__ __forceinline__ void istft_body(const KArgs& a, int64_t row, int bir,
     return cf{c.re, -c.im};
   };
 
+  // Every group runs a.run units (those past the row's end transform zeros
+  // and write nothing), so the groups of a block run the same barriers.
   for (int it = 0; it < a.run; ++it) {
     const int unit = unit0 + it;
-    if (unit >= a.units) break;  // group-uniform
+    const bool active = unit < a.units;  // group-uniform
     const int64_t s0 = 2 * (int64_t)unit;
-    const bool two = s0 + 1 < a.nseg;
+    const bool two = active && s0 + 1 < a.nseg;
     const MSIG_GLOBAL cf* f0 = zrow + s0 * nb;
     const MSIG_GLOBAL cf* f1 = f0 + nb;
     cf z[E];  // FFT input position t + TPF e
+    if (active) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int k = t + TPF * e;
-      const cf xs = bin_of(f0, k);
-      const cf xt = two ? bin_of(f1, k) : cf{0.f, 0.f};
-      // conj(X_s + i X_t)
-      z[e] = cf{xs.re - xt.im, -(xs.im + xt.re)};
+      for (int e = 0; e < E; ++e) {
+        const int k = t + TPF * e;
+        const cf xs = bin_of(f0, k);
+        const cf xt = two ? bin_of(f1, k) : cf{0.f, 0.f};
+        // conj(X_s + i X_t)
+        z[e] = cf{xs.re - xt.im, -(xs.im + xt.re)};
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) z[e] = cf{0.f, 0.f};
     }
     fft_passes<N, E, TH::LDS>(t, z, tw, lds);
     // sample n = bin(t, e): x_s = Re F / N, x_t = -Im F / N (1/N folded in w)
+    if (active) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int n = PL::bin(t, e);
-      orow[s0 * N + n] = z[e].re * w[e];
-      if (two) orow[(s0 + 1) * N + n] = -z[e].im * w[e];
+      for (int e = 0; e < E; ++e) {
+        const int n = PL::bin(t, e);
+        orow[s0 * N + n] = z[e].re * w[e];
+        if (two) orow[(s0 + 1) * N + n] = -z[e].im * w[e];
+      }
     }
     group_sync<TPF>();
   }
@@ -712,9 +784,13 @@ struct WorkItem {
 // block that a standalone launch of its job would run, so each job's result
 // is the same alone and in the batch.
 // The batch kernels run at the standalone kernels' occupancy (measured
-// faster on ragged batches than one wave fewer).
+// faster on ragged batches than one wave fewer), except at N = 2048, where
+// the job lookup's extra registers make the standalone bounds spill (9-142
+// VGPRs); there they keep two waves for welch and one for the two-signal
+// kinds.
 template <int N, int KIND, bool HALF>
 constexpr int batch_waves() {
+  if constexpr (N == 2048) return KIND == K_PSD ? 2 : 1;
   return mean_waves<N, KIND, HALF>();
 }
 

@@ -126,13 +126,56 @@ struct UnitStream {
     pair_ok = true;
   }
 
-  // HALF: the points of a unit fetched with fetch_full
-  MSIG_HD void take_full(cf* z) const {
-    static_assert(HALF, "full-unit take is the half-hop stream's");
+  // HALF: this thread's partial sum of the fetched unit's first half-block
+  // (the one it shares with the previous unit).
+  MSIG_HD float first_half_sum() const {
+    static_assert(HALF, "half-block sums are the half-hop stream's");
+    float s = 0.f;
 #pragma unroll
-    for (int e = 0; e < H; ++e) {
-      z[e] = cf{h0[e], h1[e]};
-      z[e + H] = cf{h1[e], h2[e]};
+    for (int e = 0; e < H; ++e) s += h0[e];
+    return s;
+  }
+
+  // HALF: this thread's partial sums of the fetched unit's two new
+  // half-blocks (h1, h2).
+  MSIG_HD void new_half_sums(float& pb, float& pc) const {
+    static_assert(HALF, "half-block sums are the half-hop stream's");
+    pb = 0.f;
+    pc = 0.f;
+#pragma unroll
+    for (int e = 0; e < H; ++e) pb += h1[e];
+#pragma unroll
+    for (int e = 0; e < H; ++e) pc += h2[e];
+  }
+
+  // HALF: segment means of the fetched unit from the group sums of its
+  // three half-blocks, the first carried from the previous unit.  Segment s
+  // is (h0, h1) and s+1 is (h1, h2); carry becomes the sum of h2, which is
+  // the next unit's first half.
+  static MSIG_HD void half_means(float& carry, float sb, float sc, float inv, float& mr,
+                                 float& mi) {
+    mr = (carry + sb) * inv;
+    mi = (sb + sc) * inv;
+    carry = sc;
+  }
+
+  // HALF: the points of a unit fetched with fetch_full, detrended by the
+  // segment means (mr, mi) when `detrend` and windowed by w[e] (sample
+  // t + TPF e).
+  MSIG_HD void take_full_prepped(cf* z, const float* w, bool detrend, float mr, float mi) const {
+    static_assert(HALF, "full-unit take is the half-hop stream's");
+    if (detrend) {
+#pragma unroll
+      for (int e = 0; e < H; ++e) {
+        z[e] = cf{(h0[e] - mr) * w[e], (h1[e] - mi) * w[e]};
+        z[e + H] = cf{(h1[e] - mr) * w[e + H], (h2[e] - mi) * w[e + H]};
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < H; ++e) {
+        z[e] = cf{h0[e] * w[e], h1[e] * w[e]};
+        z[e + H] = cf{h1[e] * w[e + H], h2[e] * w[e + H]};
+      }
     }
   }
 

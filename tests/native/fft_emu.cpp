@@ -262,11 +262,14 @@ static int check_stream(std::mt19937_64& rng, int nperseg, int hop, int n, int d
 }
 
 // The hop == N/2 group loop of k_mean: each group walks units
-// [unit0, min(unit0 + run, units)); units below nseg / 2 use fetch_full /
-// take_full with the next fetch clamped to the group's last full unit, and
-// the partner-less unit of an odd segment count runs once through the
-// general fetch.  Checks coverage (each unit once), that every fetch_full
-// stays inside the row, and the mean against the FP64 DFT.
+// [unit0, min(unit0 + run, units)); units below nseg / 2 use fetch_full with
+// the next fetch clamped to the group's last full unit, and take their points
+// with take_full_prepped, detrended by means built from half-block group
+// sums (first_half_sum once per group, new_half_sums per unit, half_means
+// carrying the shared half's sum), as the kernel does.  The partner-less unit
+// of an odd segment count runs once through the general fetch.  Checks
+// coverage (each unit once), that every fetch_full stays inside the row, and
+// the mean against the FP64 DFT.
 template <int N>
 static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detrend) {
   constexpr int E = N < 16 ? N : 16;
@@ -294,7 +297,13 @@ static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detren
   std::vector<int> seen(a.units, 0);
   std::vector<double> S(N, 0.0);
   bool in_bounds = true;
-  auto process = [&](std::vector<cf>& z) -> bool {
+  auto accumulate = [&](std::vector<cf>& z) -> bool {
+    std::vector<cf> Z;
+    if (!emu_fft<N, E>(table, z, Z)) return false;
+    for (int k = 0; k < N; ++k) S[k] += (double)Z[k].re * Z[k].re + (double)Z[k].im * Z[k].im;
+    return true;
+  };
+  auto process = [&](std::vector<cf>& z) -> bool {  // general prep (the tail unit)
     float sr = 0.f, si = 0.f;
     for (int i = 0; i < N; ++i) {
       sr += z[i].re;
@@ -302,11 +311,12 @@ static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detren
     }
     const float mr = detrend ? sr / N : 0.f, mi = detrend ? si / N : 0.f;
     for (int i = 0; i < N; ++i) z[i] = cf{(z[i].re - mr) * (float)wd[i], (z[i].im - mi) * (float)wd[i]};
-    std::vector<cf> Z;
-    if (!emu_fft<N, E>(table, z, Z)) return false;
-    for (int k = 0; k < N; ++k) S[k] += (double)Z[k].re * Z[k].re + (double)Z[k].im * Z[k].im;
-    return true;
+    return accumulate(z);
   };
+  using US = msig::UnitStream<N, E, false, true>;
+  std::vector<std::vector<float>> wt(TPF, std::vector<float>(E));  // window per thread register
+  for (int t = 0; t < TPF; ++t)
+    for (int e = 0; e < E; ++e) wt[t][e] = (float)wd[t + TPF * e];
   for (int g = 0; g < groups; ++g) {
     const int unit0 = g * run;
     const int u1 = unit0 + run < a.units ? unit0 + run : a.units;
@@ -323,16 +333,30 @@ static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detren
     };
     if (unit0 < uf) {
       full_fetch(unit0, true);
+      float carry = 0.f;  // group sum of the shared half-block
+      if (detrend)
+        for (int t = 0; t < TPF; ++t) carry += st[t].first_half_sum();
       for (int unit = unit0; unit < uf; ++unit) {
         const int nxt = unit + 1 < uf ? unit + 1 : uf - 1;
+        float mr = 0.f, mi = 0.f;
+        if (detrend) {
+          float sb = 0.f, sc = 0.f;
+          for (int t = 0; t < TPF; ++t) {
+            float b, c;
+            st[t].new_half_sums(b, c);
+            sb += b;
+            sc += c;
+          }
+          US::half_means(carry, sb, sc, 1.0f / N, mr, mi);
+        }
         std::vector<cf> z(N);
         for (int t = 0; t < TPF; ++t) {
           cf zt[E];
-          st[t].take_full(zt);
+          st[t].take_full_prepped(zt, wt[t].data(), detrend != 0, mr, mi);
           for (int e = 0; e < E; ++e) z[t + TPF * e] = zt[e];
         }
         full_fetch(nxt, false);
-        if (!process(z)) return 1;
+        if (!accumulate(z)) return 1;
         ++seen[unit];
       }
     }

@@ -165,6 +165,75 @@ def test_grouped_means_match_per_frame_and_oracle(rng, nperseg, detrend):
     assert_close(pxy_g, as_np(pxy_f.to(torch.complex128).mean(-2)), rtol=1e-5, atol_frac=1e-6)
 
 
+def _group_unit_counts(rows, nseg, nfft, target_blocks=1024, max_run=32):
+    """Units walked by each FFT group of a row's last block, restating the
+    launch geometry (256-thread blocks, nfft / 16 threads per group, one unit
+    per segment pair); used only to pick shapes whose last block is uneven."""
+    gpb = 256 // (nfft // 16)
+    units = (nseg + 1) // 2
+    bprt = max(1, -(-target_blocks // rows))
+    run = max(1, min(max_run, -(-units // (gpb * bprt))))
+    bpr = -(-units // (gpb * run))
+    first = (bpr - 1) * gpb * run
+    return [max(0, min(run, units - (first + g * run))) for g in range(gpb)]
+
+
+@pytest.mark.parametrize("rows,nseg", [(4, 3001), (4, 3002), (3, 999), (1, 4099), (2, 2049)])
+@pytest.mark.parametrize("detrend", [False, "constant"])
+def test_uneven_last_block_nfft2048(rng, rows, nseg, detrend):
+    """At nfft 2048 a block holds two FFT groups of two waves each, which
+    synchronise with block barriers; the last block of a row gives them
+    different unit counts (or one group none), so the kernels pad the shorter
+    group with inactive units to keep both on the same barrier sequence.  The
+    grouped means must match the per-frame kernels and the FP64 oracle, and
+    repeated calls must give bitwise identical results (a barrier mismatch
+    showed up as intermittent corruption of one unit's spectrum)."""
+    N, hop = 2048, 1024
+    counts = _group_unit_counts(rows, nseg, N)
+    n = N + hop * (nseg - 1) + 77
+    x = make_sig(rng, (rows, n))
+    y = (0.5 * x + 0.5 * make_sig(rng, (rows, n))).astype(np.float32)
+    dev = torch.device("cuda")
+    xm, ym = torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev)
+    import scipy.signal as sps
+
+    win = torch.from_numpy(sps.get_window("hann", N).astype(np.float32)).to(dev)
+    for h in (hop, 768):  # the hop == N/2 stream and the general one
+        p_grp = _kernels.rfft_power_mean(xm, win, N, h, detrend)
+        p_frm = _kernels.rfft_frames(xm, win, N, h, detrend, power=True).double().mean(-2)
+        assert_close(p_grp, as_np(p_frm), rtol=1e-5, atol_frac=1e-6)
+        c_grp = _kernels.csd_mean(xm, ym, win, N, h, detrend)
+        c_frm = _kernels.csd_frames(xm, ym, win, N, h, detrend).to(torch.complex128).mean(-2)
+        assert_close(c_grp, as_np(c_frm), rtol=1e-5, atol_frac=1e-6)
+        for _ in range(6):
+            assert torch.equal(_kernels.rfft_power_mean(xm, win, N, h, detrend), p_grp)
+            assert torch.equal(_kernels.csd_mean(xm, ym, win, N, h, detrend), c_grp)
+        pxx_g, pyy_g, pxy_g = _kernels.coherence_mean(xm, ym, win, N, h, detrend)
+        pxx_f, pyy_f, pxy_f = _kernels.coherence_frames(xm, ym, win, N, h, detrend)
+        assert_close(pxx_g, as_np(pxx_f.double().mean(-2)), rtol=1e-5, atol_frac=1e-6)
+        assert_close(pyy_g, as_np(pyy_f.double().mean(-2)), rtol=1e-5, atol_frac=1e-6)
+        assert_close(pxy_g, as_np(pxy_f.to(torch.complex128).mean(-2)), rtol=1e-5, atol_frac=1e-6)
+    if rows * n <= 4_000_000:
+        _, pw = msig.welch(x, nperseg=N, detrend=detrend)
+        assert_peak_close(pw, oracle.welch(x, nperseg=N, detrend=detrend)[1], TOL)
+    if not detrend:  # istft at nfft 2048: every group walks the same units
+        _, _, z = msig.stft(xm, nperseg=N)
+        _, xr = msig.istft(z, nperseg=N)
+        for _ in range(4):
+            assert torch.equal(msig.istft(z, nperseg=N)[1], xr)
+        err = float((xr[:, :n] - xm).abs().max() / xm.abs().max())
+        assert err < 1e-5, err
+    assert counts[0] >= 1 and len(counts) == 2
+
+
+def test_uneven_last_block_shapes_are_uneven():
+    """The shapes above do give the two groups of a row's last block
+    different unit counts (most of them; one is a control)."""
+    uneven = [(r, s) for r, s in [(4, 3001), (4, 3002), (3, 999), (1, 4099), (2, 2049)]
+              if len(set(_group_unit_counts(r, s, 2048))) > 1]
+    assert len(uneven) >= 3, uneven
+
+
 def test_grouped_single_group_and_nonlast_axis(rng):
     x_many = make_sig(rng, (2048, 320))  # nperseg=64, hop=32 -> nseg=9
     y_many = 0.5 * x_many + 0.5 * make_sig(rng, (2048, 320))
