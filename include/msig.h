@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define MSIG_ABI_VERSION 2
+#define MSIG_ABI_VERSION 3
 
 #define MSIG_EINVAL (-22)
 #define MSIG_ENOTSUP (-95)
@@ -210,6 +210,30 @@ int msig_batch_run(const msig_batch* plan, const void* device, void* workspace, 
 /* Number of kernel launches one run makes (size classes + 1). */
 int msig_batch_launches(const msig_batch* plan);
 int msig_batch_destroy(msig_batch* plan);
+
+/* One candidate tile shape for one FFT size: the msig_tuning fields at that
+ * size's index (0 selects the default). */
+typedef struct msig_tile {
+  int32_t max_run;
+  int32_t target_blocks;
+  int32_t blocks_per_cu;
+  int32_t radix_schedule;
+} msig_tile;
+
+/* Autotuner (the counterpart of the proposed msig_autotune; the reference has
+ * no tuner).  Times every candidate tile for the FFT size of the probe jobs,
+ * which must all share one nfft: each candidate is `base` (NULL: defaults)
+ * with that size's entries replaced by the candidate, planned over the probe
+ * jobs, its descriptor block copied to a device buffer the call allocates,
+ * run once untimed and then `reps` times between HIP events on `stream`.
+ * Before the first candidate the first plan runs back to back for about
+ * 60 ms so every candidate is timed at sustained clocks.  ms_out[i] receives
+ * candidate i's median ms per run and *best the index of the fastest.  The
+ * call allocates and frees device memory, writes the probe jobs' outputs,
+ * and synchronises `stream`; it must not run inside a graph capture. */
+int msig_batch_autotune(const msig_batch_job* probe, int32_t njobs, const msig_tuning* base,
+                        const msig_tile* cands, int32_t ncand, int32_t reps, float* ms_out,
+                        int32_t* best, void* stream);
 
 /* Workspace bytes for the operator-level mean entry points
  * (kind = MSIG_PSD_MEAN, MSIG_CSD_MEAN or MSIG_COH_TRIO_MEAN). */

@@ -7,6 +7,7 @@
 // _group_params / _reduce_groups :429-455.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1049,6 +1050,127 @@ int msig_batch_run(const msig_batch* plan, const void* device, void* workspace, 
 
 int msig_batch_destroy(msig_batch* plan) {
   free_plan(plan);
+  return 0;
+}
+
+int msig_batch_autotune(const msig_batch_job* probe, int32_t njobs, const msig_tuning* base,
+                        const msig_tile* cands, int32_t ncand, int32_t reps, float* ms_out,
+                        int32_t* best, void* stream) {
+  if (!probe || njobs < 1) return fail(MSIG_EINVAL, "autotuning needs at least one probe job");
+  if (!cands || ncand < 1) return fail(MSIG_EINVAL, "autotuning needs at least one candidate");
+  if (reps < 1) return fail(MSIG_EINVAL, "reps must be >= 1 (got %d)", reps);
+  if (!ms_out || !best) return fail(MSIG_EINVAL, "NULL ms_out or best");
+  const int N = probe[0].job.nfft;
+  for (int32_t i = 0; i < njobs; ++i)
+    if (probe[i].job.nfft != N)
+      return fail(MSIG_EINVAL, "probe job %d has nfft %d; all probe jobs must share nfft %d", i,
+                  probe[i].job.nfft, N);
+  if (!pow2(N) || N < MSIG_MIN_NFFT || N > MSIG_MAX_NFFT)
+    return fail(MSIG_ENOTSUP, "nfft %d outside the fused power-of-two range", N);
+  const int l2 = log2i(N);
+  for (int32_t c = 0; c < ncand; ++c) {
+    const msig_tile& k = cands[c];
+    if (k.max_run < 0 || k.target_blocks < 0 || k.blocks_per_cu < 0 || k.radix_schedule < 0 ||
+        k.radix_schedule > 1)
+      return fail(MSIG_EINVAL, "candidate %d: negative knob or radix_schedule not 0/1", c);
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // Owned resources, released on every exit.
+  struct Owned {
+    msig_batch* plan = nullptr;
+    void* desc = nullptr;
+    void* ws = nullptr;
+    size_t desc_cap = 0, ws_cap = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Owned() {
+      free_plan(plan);
+      if (desc) (void)hipFree(desc);
+      if (ws) (void)hipFree(ws);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } o;
+  hipError_t e = hipEventCreate(&o.e0);
+  if (e == hipSuccess) e = hipEventCreate(&o.e1);
+  if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
+  std::vector<char> host;
+  // plan candidate c into o.plan with its descriptor block on the device
+  auto prepare = [&](int32_t c) -> int {
+    free_plan(o.plan);
+    o.plan = nullptr;
+    msig_tuning t;
+    if (base) t = *base;
+    else std::memset(&t, 0, sizeof t);
+    t.max_run[l2] = cands[c].max_run;
+    t.target_blocks[l2] = cands[c].target_blocks;
+    t.blocks_per_cu[l2] = cands[c].blocks_per_cu;
+    t.radix_schedule[l2] = cands[c].radix_schedule;
+    int rc = msig_batch_create(probe, njobs, &t, &o.plan);
+    if (rc) return fail(rc, "candidate %d: %s", c, msig_last_error());
+    hipError_t err;
+    if (o.plan->desc_bytes > o.desc_cap) {
+      if (o.desc) (void)hipFree(o.desc);
+      o.desc = nullptr;
+      err = hipMalloc(&o.desc, o.plan->desc_bytes);
+      if (err != hipSuccess) return hip_fail(err, "hipMalloc(descriptor block)");
+      o.desc_cap = o.plan->desc_bytes;
+    }
+    const size_t wsb = o.plan->ws_bytes > 0 ? o.plan->ws_bytes : 256;
+    if (wsb > o.ws_cap) {
+      if (o.ws) (void)hipFree(o.ws);
+      o.ws = nullptr;
+      err = hipMalloc(&o.ws, wsb);
+      if (err != hipSuccess) return hip_fail(err, "hipMalloc(workspace)");
+      o.ws_cap = wsb;
+    }
+    host.assign(o.plan->desc_bytes, 0);
+    rc = msig_batch_pack(o.plan, host.data(), host.size(), o.desc);
+    if (rc) return rc;
+    // the previous candidate's runs may still read the block: copy in stream order
+    err = hipMemcpyAsync(o.desc, host.data(), host.size(), hipMemcpyHostToDevice, s);
+    if (err == hipSuccess) err = hipStreamSynchronize(s);  // pageable source: keep it alive
+    if (err != hipSuccess) return hip_fail(err, "descriptor copy");
+    return 0;
+  };
+  auto run = [&]() -> int { return msig_batch_run(o.plan, o.desc, o.ws, o.ws_cap, stream); };
+  auto elapsed = [&](float* ms) -> int {
+    hipError_t err = hipEventSynchronize(o.e1);
+    if (err == hipSuccess) err = hipEventElapsedTime(ms, o.e0, o.e1);
+    return err == hipSuccess ? 0 : hip_fail(err, "hipEventElapsedTime");
+  };
+  int rc = prepare(0);
+  if (rc) return rc;
+  // bring the clocks to their sustained level: ~60 ms of back-to-back runs
+  float warm = 0.f;
+  for (int it = 0; it < 64 && warm < 60.f; ++it) {
+    e = hipEventRecord(o.e0, s);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    for (int k = 0; k < 8; ++k)
+      if ((rc = run())) return rc;
+    e = hipEventRecord(o.e1, s);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    float ms = 0.f;
+    if ((rc = elapsed(&ms))) return rc;
+    warm += ms;
+  }
+  int32_t arg = 0;
+  for (int32_t c = 0; c < ncand; ++c) {
+    if (c > 0 && (rc = prepare(c))) return rc;
+    if ((rc = run())) return rc;  // untimed
+    std::vector<float> ts((size_t)reps);
+    for (int32_t r = 0; r < reps; ++r) {
+      e = hipEventRecord(o.e0, s);
+      if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+      if ((rc = run())) return rc;
+      e = hipEventRecord(o.e1, s);
+      if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+      if ((rc = elapsed(&ts[(size_t)r]))) return rc;
+    }
+    std::sort(ts.begin(), ts.end());
+    ms_out[c] = ts[(size_t)reps / 2];
+    if (ms_out[c] < ms_out[arg]) arg = c;
+  }
+  *best = arg;
   return 0;
 }
 

@@ -17,7 +17,7 @@ import os
 
 # MSIG_LIBRARY selects another build of the library (e.g. a tuning variant)
 LIB_PATH = Path(os.environ.get("MSIG_LIBRARY") or Path(__file__).with_name("libmsig.so"))
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 MSIG_EINVAL = -22
 MSIG_ENOTSUP = -95
@@ -79,6 +79,17 @@ class MsigTuning(ctypes.Structure):
     ]
 
 
+class MsigTile(ctypes.Structure):
+    """One candidate tile shape for one FFT size (msig_tuning's fields at it)."""
+
+    _fields_ = [
+        ("max_run", ctypes.c_int32),
+        ("target_blocks", ctypes.c_int32),
+        ("blocks_per_cu", ctypes.c_int32),
+        ("radix_schedule", ctypes.c_int32),
+    ]
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -110,6 +121,10 @@ SIGNATURES = {
     "msig_batch_run": (ctypes.c_int, [_vp, _vp, _vp, _sz, _vp]),
     "msig_batch_launches": (ctypes.c_int, [_vp]),
     "msig_batch_destroy": (ctypes.c_int, [_vp]),
+    "msig_batch_autotune": (ctypes.c_int, [ctypes.POINTER(MsigBatchJob), _i32,
+                                           ctypes.POINTER(MsigTuning), ctypes.POINTER(MsigTile),
+                                           _i32, _i32, ctypes.POINTER(ctypes.c_float),
+                                           ctypes.POINTER(_i32), _vp]),
     "msig_istft_frames": (ctypes.c_int, [_vp, _i64, _i32, _i32, _vp, _vp, _vp]),
     "msig_overlap_add": (ctypes.c_int, [_vp, _i64, _i32, _i32, _i32, _i32, _vp, _i64, _i64, _vp, _vp]),
     "msig_median_frames": (ctypes.c_int, [_vp, _i64, _i32, _i32, _i32, ctypes.c_double, _i32, _i32, _vp, _vp]),

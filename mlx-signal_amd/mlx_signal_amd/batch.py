@@ -86,6 +86,7 @@ class SpectralBatch:
         self.bytes = 0
         for i, r in enumerate(requests):
             jobs[i] = self._plan(r)
+        self.jobs = jobs  # the planned msig_batch_job array (kept alive with its tensors)
         self._tuning = tuning_struct(tuning)
         handle = ctypes.c_void_p()
         tp = ctypes.byref(self._tuning) if self._tuning is not None else None

@@ -14,24 +14,29 @@ the tile shape of a job's decomposition:
   first and the remainder radix last, 1 the remainder radix first (a
   separate kernel only where nfft is not a power of 16).
 
-The chosen table is cached per device name in a JSON file so later processes
+The timing loop is the library's own (``msig_batch_autotune``): it plans
+every candidate over the probe jobs, warms the clocks, and times runs between
+HIP events on the stream, so any host of the C ABI tunes the same way.  The
+chosen table is cached per device name in a JSON file so later processes
 reuse it.  Jobs are planned with their size's tuned knobs both alone and in a
 batch, so tuning never breaks the alone-equals-in-mix property.
 """
 
 from __future__ import annotations
 
+import ctypes
 import itertools
 import json
 import os
-import statistics
 from pathlib import Path
 
 import torch
 
+from . import _hip
 from .batch import SpectralBatch, SpectralRequest
 
-__all__ = ["autotune", "cached_tuning", "candidates_for", "load_tuning", "save_tuning", "CANDIDATES"]
+__all__ = ["autotune", "cached_tuning", "candidates_for", "load_tuning", "save_tuning",
+           "time_candidates", "CANDIDATES"]
 
 CANDIDATES = [
     dict(target_blocks=tb, max_run=mr, blocks_per_cu=bpc)
@@ -102,19 +107,26 @@ def _probe_requests(nfft, dev, gen):
     return reqs
 
 
-def _time_batch(b: SpectralBatch, reps: int) -> float:
-    b.run()
-    torch.cuda.synchronize()
-    ts = []
-    for _ in range(reps):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        b.run()
-        e1.record()
-        torch.cuda.synchronize()
-        ts.append(e0.elapsed_time(e1))
-    return statistics.median(ts)
+def time_candidates(requests, nfft, candidates, reps=5):
+    """Median ms per batch run of `requests` (all of FFT size nfft) under each
+    candidate knob dict, timed by the library (msig_batch_autotune)."""
+    lib = _hip.load()
+    b = SpectralBatch(requests)
+    try:
+        n = len(candidates)
+        tiles = (_hip.MsigTile * n)()
+        for i, c in enumerate(candidates):
+            tiles[i] = _hip.MsigTile(int(c.get("max_run", 0)), int(c.get("target_blocks", 0)),
+                                     int(c.get("blocks_per_cu", 0)), int(c.get("radix_schedule", 0)))
+        ms = (ctypes.c_float * n)()
+        best = ctypes.c_int32(-1)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(b.device).cuda_stream)
+        rc = lib.msig_batch_autotune(b.jobs, len(requests), None, tiles, n, int(reps), ms,
+                                     ctypes.byref(best), stream)
+        _hip.check(rc, "batch autotune")
+        return [float(v) for v in ms], int(best.value)
+    finally:
+        b.close()
 
 
 def autotune(nffts=(256, 1024, 4096), reps=5, seed=0, candidates=None, save=True):
@@ -126,14 +138,11 @@ def autotune(nffts=(256, 1024, 4096), reps=5, seed=0, candidates=None, save=True
     report = {}
     for nfft in nffts:
         reqs = _probe_requests(nfft, dev, gen)
-        timings = []
-        for cand in candidates_for(nfft, candidates):
-            b = SpectralBatch(reqs, tuning={nfft: cand})
-            timings.append((_time_batch(b, reps), cand))
-            b.close()
-        timings.sort(key=lambda tc: tc[0])
-        best[nfft] = dict(timings[0][1])
-        report[nfft] = [(round(t, 4), c) for t, c in timings[:3]]
+        cands = candidates_for(nfft, candidates)
+        ms, i = time_candidates(reqs, nfft, cands, reps)
+        best[nfft] = dict(cands[i])
+        order = sorted(range(len(cands)), key=lambda k: ms[k])
+        report[nfft] = [(round(ms[k], 4), cands[k]) for k in order[:3]]
         del reqs
     if save:
         save_tuning(best, dev)

@@ -86,3 +86,36 @@ def test_tuning_struct_matches_header():
     assert ctypes.sizeof(_hip.MsigTuning) == 4 * 13 * len(fields)
     abi = int(re.search(r"#define MSIG_ABI_VERSION (\d+)", text).group(1))
     assert abi == _hip.ABI_VERSION
+
+
+def test_tile_struct_matches_header():
+    """msig_tile's fields, in order, are the ctypes structure's."""
+    text = (ROOT / "include" / "msig.h").read_text()
+    body = re.search(r"typedef struct msig_tile \{(.*?)\} msig_tile;", text, re.S).group(1)
+    fields = re.findall(r"int32_t (\w+);", body)
+    assert fields == [f for f, _ in _hip.MsigTile._fields_]
+    assert ctypes.sizeof(_hip.MsigTile) == 4 * len(fields)
+
+
+def test_autotune_rejects_bad_arguments_without_gpu():
+    """msig_batch_autotune validates its arguments before any device call."""
+    if not _hip.LIB_PATH.exists():
+        pytest.skip("libmsig.so not built")
+    lib = _hip.load()
+
+    def job(nfft):
+        j = _hip.MsigJob(_hip.PSD_MEAN, _hip.F_ONESIDED, 1, 4096, 4096, 4096, nfft, nfft // 2,
+                         nfft, (4096 - nfft) // (nfft // 2) + 1, 1.0)
+        return _hip.MsigBatchJob(j, 1, None, 1, 1, None, None)  # never dereferenced here
+
+    jobs = (_hip.MsigBatchJob * 2)(job(256), job(1024))
+    tiles = (_hip.MsigTile * 1)(_hip.MsigTile(8, 1024, 4, 0))
+    ms = (ctypes.c_float * 1)()
+    best = ctypes.c_int32(-1)
+    rc = lib.msig_batch_autotune(jobs, 2, None, tiles, 1, 3, ms, ctypes.byref(best), None)
+    assert rc == _hip.MSIG_EINVAL and b"share nfft" in lib.msig_last_error()
+    assert lib.msig_batch_autotune(jobs, 1, None, tiles, 0, 3, ms, ctypes.byref(best), None) == _hip.MSIG_EINVAL
+    assert lib.msig_batch_autotune(jobs, 1, None, tiles, 1, 0, ms, ctypes.byref(best), None) == _hip.MSIG_EINVAL
+    bad = (_hip.MsigTile * 1)(_hip.MsigTile(8, 1024, 4, 2))
+    assert lib.msig_batch_autotune(jobs, 1, None, bad, 1, 3, ms, ctypes.byref(best), None) == _hip.MSIG_EINVAL
+    assert best.value == -1

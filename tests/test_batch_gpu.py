@@ -120,3 +120,31 @@ def test_batch_radix_schedule_matches_oracle(rng):
     outs = [o.clone() for o in SpectralBatch(reqs, tuning=tuning).run()]
     for (kind, x, y, kw), out in zip(host, outs):
         assert_peak_close(out, _oracle(kind, x, y, kw), TOL)
+
+
+def test_native_autotune_times_every_candidate(rng):
+    """msig_batch_autotune (through tuning.time_candidates) returns a positive
+    median per candidate and the index of the smallest; the winner's plan runs
+    and matches the oracle."""
+    from mlx_signal_amd.tuning import candidates_for, time_candidates
+
+    dev = torch.device("cuda")
+    reqs, host = [], []
+    for ch, n, kind in ((2, 20000, "welch"), (3, 9000, "csd"), (1, 70000, "welch")):
+        x = make_sig(rng, (ch, n))
+        y = (0.5 * x + make_sig(rng, (ch, n))).astype(np.float32) if kind == "csd" else None
+        host.append((kind, x, y, dict(fs=1.0, window="hann", nperseg=1024)))
+        reqs.append(SpectralRequest(kind, torch.from_numpy(x).to(dev),
+                                    None if y is None else torch.from_numpy(y).to(dev), nperseg=1024))
+    cands = candidates_for(1024, [dict(target_blocks=256, max_run=8, blocks_per_cu=2),
+                                  dict(target_blocks=1024, max_run=32, blocks_per_cu=4),
+                                  dict(target_blocks=4096, max_run=8, blocks_per_cu=8)])
+    assert len(cands) == 6  # 1024 = 16 * 16 * 4: two radix schedules
+    ms, best = time_candidates(reqs, 1024, cands, reps=3)
+    assert len(ms) == len(cands) and all(t > 0 for t in ms)
+    assert best == min(range(len(ms)), key=lambda i: ms[i])
+    b = SpectralBatch(reqs, tuning={1024: cands[best]})
+    b.run()
+    for out, (kind, x, y, kw) in zip(b.outputs, host):
+        assert_peak_close(out, _oracle(kind, x, y, kw), TOL)
+    b.close()
