@@ -344,8 +344,16 @@ __global__ __launch_bounds__(MED_BT) void k_median(MedArgs a) {
   uint32_t* kmax_s = kmin_s + MED_MAX_CPB;
   const int tid = threadIdx.x;
   const int col = tid / TPC, lt = tid % TPC;
-  const int64_t row = blockIdx.x / a.ncb;
-  const int col0 = (int)(blockIdx.x - row * a.ncb) * CPB;
+  // Blocks are dispatched round-robin over the 8 XCDs (block b to XCD b % 8,
+  // as that XCD's (b / 8)-th block).  Each XCD is given a contiguous range
+  // of (row, column-block) tiles, so neighbouring column blocks of a row,
+  // which read the same cache lines of every segment, share one L2 instead
+  // of each XCD fetching the lines for a few of their bytes.
+  const uint32_t nb = gridDim.x, bx = blockIdx.x;
+  const uint32_t per = nb / 8u, rem = nb % 8u, xcd = bx % 8u;
+  const uint32_t tile_id = xcd * per + (xcd < rem ? xcd : rem) + bx / 8u;
+  const int64_t row = tile_id / a.ncb;
+  const int col0 = (int)(tile_id - row * a.ncb) * CPB;
   const int ncol = a.cols - col0 < CPB ? a.cols - col0 : CPB;  // columns present
   const MSIG_GLOBAL float* src = as_global(a.in) + row * (int64_t)nseg * a.cols + col0;
 
