@@ -126,13 +126,13 @@ struct TwiddleHome {
 // Which kernels want the LDS table.  Up to N = 2048 (tables <= 16 KB) all
 // do.  At N = 4096 the 32 KB table pays only where the VGPRs it frees let
 // the kernel run two waves per SIMD instead of one, which is the two-signal
-// hop == N/2 mean (measured 358 -> 273 us on 64 x 2^20 csd); the others
-// stay at the same occupancy either way and only pay for the extra LDS
-// reads (welch 125 -> 136 us, coherence 363 -> 413 us), so they keep
-// register twiddles.
+// hop == N/2 means (measured on 64 x 2^20: csd 358 -> 273 us, coherence
+// 362 -> 251 us); the others stay at the same occupancy either way and only
+// pay for the extra LDS reads (welch 125 -> 136 us, one-wave coherence
+// 363 -> 413 us), so they keep register twiddles.
 template <int N, int KIND, bool HALF>
 constexpr bool mean_twl() {
-  return N < 4096 || (KIND == K_CSD && HALF);
+  return N < 4096 || (KIND != K_PSD && HALF);
 }
 template <int N>
 constexpr bool frames_twl() {
@@ -363,12 +363,14 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
         if constexpr (KIND == K_PSD) {
           transform_acc(zx, zy, true, [] {});
         } else {
-          const UnitStream<N, E, CPLX, HALF> ycur = sy;  // this unit's raw y halves
-          sy.fetch_full(nxt, false);
+          // y's points are built after x's FFT, and y's next loads are issued
+          // then, so x's FFT holds one set of y half-blocks, not two (what
+          // lets hop == N/2 coherence run two waves per SIMD)
           transform_acc(zx, zy, true, [&] {
             float mry, miy;
-            means(ycur, syc, mry, miy);
-            ycur.take_full_prepped(zy, w, a.detrend, mry, miy);
+            means(sy, syc, mry, miy);
+            sy.take_full_prepped(zy, w, a.detrend, mry, miy);
+            sy.fetch_full(nxt, false);
           });
         }
       }
@@ -481,15 +483,18 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
 // whose deduplicated 16 KB twiddle table leaves LDS for three blocks) and
 // two above; two-signal kinds hold both spectra and their prefetch buffers
 // and run with two (CSD; at N = 4096 only the hop == N/2 form, whose
-// twiddles move to LDS) or one (coherence).
+// twiddles move to LDS) or one (coherence, except the hop == N/2 form: two
+// waves with 14-34 spilled VGPRs measured 281 -> 183 us on 64 x 2^20 at
+// N = 1024, 364 -> 231 us at 2048 and 362 -> 251 us at 4096 (twiddles in
+// LDS there), where one wave left the latencies exposed).
 template <int N, int KIND, bool HALF>
 constexpr int mean_waves() {
   if constexpr (N >= 2048) {
     if constexpr (KIND == K_PSD) return (N == 2048 && HALF) ? 3 : 2;
     if constexpr (KIND == K_CSD) return (N < 4096 || HALF) ? 2 : 1;
-    return 1;
+    return HALF ? 2 : 1;
   }
-  return KIND == K_PSD ? 3 : (KIND == K_CSD ? 2 : 1);
+  return KIND == K_PSD ? 3 : (KIND == K_CSD ? 2 : (HALF ? 2 : 1));
 }
 template <int N, int OK>
 constexpr int frames_waves() {
@@ -784,13 +789,15 @@ struct WorkItem {
 // block that a standalone launch of its job would run, so each job's result
 // is the same alone and in the batch.
 // The batch kernels run at the standalone kernels' occupancy (measured
-// faster on ragged batches than one wave fewer), except at N = 2048, where
-// the job lookup's extra registers make the standalone bounds spill (9-142
-// VGPRs); there they keep two waves for welch and one for the two-signal
-// kinds.
+// faster on ragged batches than one wave fewer), except where the job
+// lookup's extra registers make the standalone bounds spill heavily: at
+// N = 2048 (9-142 VGPRs) they keep two waves for welch and one for the
+// two-signal kinds, and coherence keeps one wave everywhere (47-88 VGPRs at
+// two).
 template <int N, int KIND, bool HALF>
 constexpr int batch_waves() {
   if constexpr (N == 2048) return KIND == K_PSD ? 2 : 1;
+  if constexpr (KIND == K_COH) return 1;
   return mean_waves<N, KIND, HALF>();
 }
 
