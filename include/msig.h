@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define MSIG_ABI_VERSION 3
+#define MSIG_ABI_VERSION 4
 
 #define MSIG_EINVAL (-22)
 #define MSIG_ENOTSUP (-95)
@@ -85,6 +85,13 @@ typedef struct msig_job {
   int32_t nseg;    /* (n - nperseg) / nstep + 1, checked                        */
   double scale;    /* multiplies every mean/power output bin; complex per-frame
                       outputs are multiplied by sqrt(scale)                     */
+  int64_t lead;    /* per-frame kinds only (0 for means and batches): zero
+                      samples virtually in front of each row, so that `n` is
+                      the length of the extended row the segments are cut
+                      from (stft's boundary='zeros' extension plus padding)    */
+  int64_t nvalid;  /* samples stored per row (0: n - lead).  Extended sample p
+                      is stored sample p - lead, and reads as zero outside
+                      [0, nvalid); ld_x / ld_y >= nvalid.                       */
 } msig_job;
 
 /* ABI version the library was built with (MSIG_ABI_VERSION). */

@@ -519,13 +519,13 @@ static hipError_t dispatch_mean(int N, int kk, bool cplx, const KArgs& a, int64_
   return hipErrorInvalidValue;
 }
 
-static hipError_t dispatch_frames(int N, int ok, bool cplx, const KArgs& a, int64_t rows,
-                                  hipStream_t s) {
+static hipError_t dispatch_frames(int N, int ok, bool cplx, bool ext, const KArgs& a,
+                                  int64_t rows, hipStream_t s) {
   const bool half = half_hop(a, N, cplx);
   switch (N) {
 #define C(NV) \
   case NV:    \
-    return launch_frames_##NV(ok, cplx, half, a, rows, s);
+    return launch_frames_##NV(ok, cplx, half, ext, a, rows, s);
     C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
 #undef C
   }
@@ -577,6 +577,9 @@ static bool is_pair(int kind) {
 
 static int check_precision(const msig_job* j);
 
+// samples stored per row (the extended row's length without the virtual zeros)
+static int64_t stored_len(const msig_job* j) { return j->nvalid ? j->nvalid : j->n - j->lead; }
+
 static int check_job(const msig_job* j) {
   if (!j) return fail(MSIG_EINVAL, "job is NULL");
   if (j->kind < MSIG_PSD_MEAN || j->kind > MSIG_COH_FRAMES)
@@ -588,8 +591,17 @@ static int check_job(const msig_job* j) {
     return fail(MSIG_EINVAL, "nfft (%d) must be >= nperseg (%d)", j->nfft, j->nperseg);
   if (j->n < j->nperseg)
     return fail(MSIG_EINVAL, "n (%lld) must be >= nperseg (%d)", (long long)j->n, j->nperseg);
-  if (j->ld_x < j->n) return fail(MSIG_EINVAL, "ld_x < n");
-  if (is_pair(j->kind) && j->ld_y < j->n) return fail(MSIG_EINVAL, "ld_y < n");
+  if (j->lead < 0 || j->nvalid < 0) return fail(MSIG_EINVAL, "lead and nvalid must be >= 0");
+  const bool ext = j->lead != 0 || j->nvalid != 0;
+  if (ext && j->kind != MSIG_FRAMES_COMPLEX)
+    return fail(MSIG_ENOTSUP,
+                "the virtual zero extension (lead / nvalid) is for complex per-frame spectra");
+  const int64_t stored = stored_len(j);
+  if (stored < 1 || j->lead + stored > j->n)
+    return fail(MSIG_EINVAL, "lead (%lld) + nvalid (%lld) must lie in [1, n = %lld]",
+                (long long)j->lead, (long long)stored, (long long)j->n);
+  if (j->ld_x < stored) return fail(MSIG_EINVAL, "ld_x < stored samples per row");
+  if (is_pair(j->kind) && j->ld_y < stored) return fail(MSIG_EINVAL, "ld_y < stored samples per row");
   const int64_t nseg = (j->n - j->nperseg) / j->nstep + 1;
   if (nseg != j->nseg)
     return fail(MSIG_EINVAL, "nseg %d != (n - nperseg) / nstep + 1 = %lld", j->nseg,
@@ -657,6 +669,8 @@ static KArgs base_args(const msig_job* j, const void* x, const void* y, const fl
   a.run = g.run;
   a.bpr = g.bpr;
   a.detrend = (j->flags & MSIG_F_DETREND_CONST) ? 1 : 0;
+  a.lead = j->lead;
+  a.nvalid = stored_len(j);
   return a;
 }
 
@@ -817,7 +831,9 @@ int msig_spectral_frames(const msig_job* job, const void* x, const void* y, cons
   }
   a.oscale = ok == F_COMPLEX ? (float)std::sqrt(job->scale) : (float)job->scale;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipError_t e = timed_launch(s, [&] { return dispatch_frames(N, ok, cplx, a, job->rows, s); });
+  const bool ext = job->lead != 0 || job->nvalid != 0;
+  hipError_t e =
+      timed_launch(s, [&] { return dispatch_frames(N, ok, cplx, ext, a, job->rows, s); });
   if (e != hipSuccess) return hip_fail(e, "fused frames kernel launch");
   return 0;
 }

@@ -539,7 +539,7 @@ __device__ __forceinline__ void separate(int t, const cf* z, cf* lds, cf* x0, cf
   group_sync<TPF>();
 }
 
-template <int N, int E, int OK, bool CPLX, bool HALF>
+template <int N, int E, int OK, bool CPLX, bool HALF, bool EXT = false>
 __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir, cf* lds_all,
                                             float* red, const cf* twl) {
   using PL = Plan<N, E>;
@@ -567,7 +567,7 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
     w[e] = n < a.nperseg ? as_global(a.win)[n] : 0.f;
   }
   const size_t esz = CPLX ? sizeof(cf) : sizeof(float);
-  UnitStream<N, E, CPLX, HALF> sx, sy;
+  UnitStream<N, E, CPLX, HALF, EXT> sx, sy;
   sx.row = static_cast<const char*>(a.x) + (size_t)row * a.ld_x * esz;
   sx.t = t;
   if constexpr (PAIR) {
@@ -667,7 +667,8 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
   }
 }
 
-template <int N, int E, int OK, bool CPLX, bool HALF>
+// EXT: rows read through the virtual zero extension (stft's boundary zeros)
+template <int N, int E, int OK, bool CPLX, bool HALF, bool EXT = false>
 __global__ __launch_bounds__(BT, (frames_waves<N, OK>())) void k_frames(const KArgs a) {
   using PL = Plan<N, E>;
   using TH = TwiddleHome<N, E, 0, frames_twl<N>()>;
@@ -678,7 +679,8 @@ __global__ __launch_bounds__(BT, (frames_waves<N, OK>())) void k_frames(const KA
     fill_twiddle_lds<N, E>(twl, a.twt);
     __syncthreads();
   }
-  frames_body<N, E, OK, CPLX, HALF>(a, blockIdx.x / a.bpr, blockIdx.x % a.bpr, lds_all, red, twl);
+  frames_body<N, E, OK, CPLX, HALF, EXT>(a, blockIdx.x / a.bpr, blockIdx.x % a.bpr, lds_all, red,
+                                         twl);
 }
 
 

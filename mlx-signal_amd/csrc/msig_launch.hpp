@@ -65,8 +65,8 @@ hipError_t launch_mean_mfma256(int prec, const KArgs& a, int64_t rows, hipStream
 #define MSIG_DECLARE_LAUNCHERS(NV)                                                         \
   hipError_t launch_mean_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows,  \
                               hipStream_t s);                                              \
-  hipError_t launch_frames_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows, \
-                                hipStream_t s);                                             \
+  hipError_t launch_frames_##NV(int kind, bool cplx, bool half, bool ext, const KArgs& a,    \
+                                int64_t rows, hipStream_t s);                               \
   hipError_t launch_mean_batch_##NV(int kind, bool cplx, bool half, int sched,             \
                                     const KArgs* jobs, const WorkItem* queue, int32_t nwork, \
                                     float* ws, int grid, hipStream_t s);                    \
@@ -90,6 +90,15 @@ MSIG_DECLARE_LAUNCHERS(4096)
   hipLaunchKernelGGL((k_mean<NV, (NV < 16 ? NV : 16), K, C, H>), grid, block, 0, s, a)
 #define MSIG_LAUNCH_FRAMES(NV, K, C, H) \
   hipLaunchKernelGGL((k_frames<NV, (NV < 16 ? NV : 16), K, C, H>), grid, block, 0, s, a)
+// complex frames through the virtual zero extension (ext) or plain
+#define MSIG_LAUNCH_FRAMES_COMPLEX(NV, C, H)                                                    \
+  do {                                                                                        \
+    if (ext)                                                                                  \
+      hipLaunchKernelGGL((k_frames<NV, (NV < 16 ? NV : 16), F_COMPLEX, C, H, true>), grid, block, \
+                         0, s, a);                                                            \
+    else                                                                                      \
+      MSIG_LAUNCH_FRAMES(NV, F_COMPLEX, C, H);                                                \
+  } while (0)
 #define MSIG_LAUNCH_MEAN_BATCH_S(NV, K, C, H, SCH)                                   \
   hipLaunchKernelGGL((k_mean_batch<NV, (NV < 16 ? NV : 16), K, C, H, SCH>), dim3(grid),  \
                      dim3(BT), 0, s, jobs, queue, nwork, ws)
@@ -121,17 +130,17 @@ MSIG_DECLARE_LAUNCHERS(4096)
     }                                                                                      \
     return hipGetLastError();                                                              \
   }                                                                                        \
-  hipError_t launch_frames_##NV(int kind, bool cplx, bool half, const KArgs& a,            \
+  hipError_t launch_frames_##NV(int kind, bool cplx, bool half, bool ext, const KArgs& a,  \
                                 int64_t rows, hipStream_t s) {                             \
     const dim3 grid((unsigned)(rows * a.bpr)), block(BT);                                  \
     if (cplx) {                                                                            \
-      if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES(NV, F_COMPLEX, true, false);               \
+      if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES_COMPLEX(NV, true, false);                  \
       else MSIG_KIND_SWITCH(MSIG_LAUNCH_FRAMES, NV, F_POWER, F_CSD, F_COH, true, false)    \
     } else if (half) {                                                                     \
-      if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES(NV, F_COMPLEX, false, true);               \
+      if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES_COMPLEX(NV, false, true);                  \
       else MSIG_KIND_SWITCH(MSIG_LAUNCH_FRAMES, NV, F_POWER, F_CSD, F_COH, false, true)    \
     } else {                                                                               \
-      if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES(NV, F_COMPLEX, false, false);              \
+      if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES_COMPLEX(NV, false, false);                 \
       else MSIG_KIND_SWITCH(MSIG_LAUNCH_FRAMES, NV, F_POWER, F_CSD, F_COH, false, false)   \
     }                                                                                      \
     return hipGetLastError();                                                              \

@@ -26,6 +26,8 @@ struct KArgs {
   int32_t apply_factor;
   float oscale;      // frames: power scale or sqrt(scale) for complex output
   int64_t ws_off;    // batch plans: this job's partials start at ws_base + ws_off
+  int64_t lead;      // zero samples virtually in front of each row (frames kinds)
+  int64_t nvalid;    // samples stored per row; the extended row reads zero past them
 };
 
 // Raw samples of consecutive units of one row, fetched one unit ahead so the
@@ -37,7 +39,14 @@ struct KArgs {
 // h_{2u+2} is also the first half of the next pair, so each sample is loaded
 // into registers exactly once per group.  Units past the row's end, and the
 // missing partner of an odd segment count, load zeros.
-template <int N, int E, bool CPLX, bool HALF>
+//
+// EXT: virtual zero extension (the complex per-frame kernel, KArgs::lead /
+// nvalid): sample p of the extended row is stored sample p - lead, and reads
+// as zero outside [0, nvalid).  Fetches whose span lies inside the stored
+// samples take the unmasked loads; only the spans that cross an edge test
+// each sample.  EXT = false compiles the plain fetches, so the other kernels
+// carry none of this.
+template <int N, int E, bool CPLX, bool HALF, bool EXT = false>
 struct UnitStream {
   static_assert(!(CPLX && HALF), "half-hop reuse is for real input");
   static constexpr int TPF = N / E;
@@ -59,6 +68,33 @@ struct UnitStream {
     }
   }
 
+  // Offsets [lo, hi) of a span of len samples starting at stored index start
+  // (uniform across the group) that are stored samples.
+  static MSIG_HD void stored_window(int64_t start, int len, int64_t nvalid, int& lo, int& hi) {
+    const int64_t a = -start, b = nvalid - start;
+    lo = (int)(a < 0 ? 0 : (a > len ? len : a));
+    hi = (int)(b < 0 ? 0 : (b > len ? len : b));
+  }
+
+  // half_load on the virtually extended row (stored samples [0, nvalid) at
+  // extended positions [lead, lead + nvalid))
+  MSIG_HD void half_load_ext(float* dst, int64_t h, bool valid, int64_t lead, int64_t nvalid) {
+    const int64_t start = h * (N / 2) - lead;  // stored index of the half's first sample
+    const MSIG_GLOBAL float* p = as_global(reinterpret_cast<const float*>(row)) + start;
+    int lo, hi;
+    stored_window(start, N / 2, nvalid, lo, hi);
+    if (valid && lo == 0 && hi == N / 2) {
+#pragma unroll
+      for (int e = 0; e < H; ++e) dst[e] = p[t + TPF * e];
+      return;
+    }
+#pragma unroll
+    for (int e = 0; e < H; ++e) {
+      const int o = t + TPF * e;
+      dst[e] = (valid && o >= lo && o < hi) ? p[o] : 0.f;
+    }
+  }
+
   // first == true: nothing carried from a previous fetch
   MSIG_HD void fetch(int unit, bool first, const KArgs& a) {
     const bool active = unit < a.units;
@@ -67,26 +103,35 @@ struct UnitStream {
       const bool v1 = active && (s0 + 1 < a.nseg);
       pair_ok = v1;
       if (first) {
-        half_load(h0, s0, active);
+        if constexpr (EXT) half_load_ext(h0, s0, active, a.lead, a.nvalid);
+        else half_load(h0, s0, active);
       } else {
 #pragma unroll
         for (int e = 0; e < H; ++e) h0[e] = active ? h2[e] : 0.f;
       }
-      half_load(h1, s0 + 1, active);
-      half_load(h2, s0 + 2, v1);
+      if constexpr (EXT) {
+        half_load_ext(h1, s0 + 1, active, a.lead, a.nvalid);
+        half_load_ext(h2, s0 + 2, v1, a.lead, a.nvalid);
+      } else {
+        half_load(h1, s0 + 1, active);
+        half_load(h2, s0 + 2, v1);
+      }
     } else if constexpr (!CPLX) {
       const MSIG_GLOBAL float* xrow = as_global(reinterpret_cast<const float*>(row));
       const int64_t s0 = 2 * (int64_t)unit;
       const bool v1 = active && (s0 + 1 < a.nseg);
-      const MSIG_GLOBAL float* p0 = xrow + s0 * a.hop;
+      const int64_t lead = EXT ? a.lead : 0;
+      const int64_t i0 = s0 * a.hop - lead;  // stored index of segment s0's first sample
+      const MSIG_GLOBAL float* p0 = xrow + i0;
       const MSIG_GLOBAL float* p1 = p0 + a.hop;
-      if (a.nperseg == N && v1) {
+      const bool inside = !EXT || (i0 >= 0 && i0 + a.hop + a.nperseg <= a.nvalid);
+      if (a.nperseg == N && v1 && inside) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           raw[e].re = p0[t + TPF * e];
           raw[e].im = p1[t + TPF * e];
         }
-      } else {
+      } else if (inside) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const int n = t + TPF * e;
@@ -94,13 +139,26 @@ struct UnitStream {
           raw[e].re = (active && in) ? p0[n] : 0.f;
           raw[e].im = (v1 && in) ? p1[n] : 0.f;
         }
+      } else {
+        int loa, hia, lob, hib;
+        stored_window(i0, a.nperseg, a.nvalid, loa, hia);
+        stored_window(i0 + a.hop, a.nperseg, a.nvalid, lob, hib);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int n = t + TPF * e;
+          raw[e].re = (active && n >= loa && n < hia) ? p0[n] : 0.f;
+          raw[e].im = (v1 && n >= lob && n < hib) ? p1[n] : 0.f;
+        }
       }
     } else {
-      const MSIG_GLOBAL cf* p0 = as_global(reinterpret_cast<const cf*>(row)) + (int64_t)unit * a.hop;
+      const int64_t i0 = (int64_t)unit * a.hop - (EXT ? a.lead : 0);
+      const MSIG_GLOBAL cf* p0 = as_global(reinterpret_cast<const cf*>(row)) + i0;
+      int lo = 0, hi = a.nperseg;
+      if constexpr (EXT) stored_window(i0, a.nperseg, a.nvalid, lo, hi);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int n = t + TPF * e;
-        if (active && n < a.nperseg) {
+        if (active && n < a.nperseg && (!EXT || (n >= lo && n < hi))) {
           raw[e].re = p0[n].re;
           raw[e].im = p0[n].im;
         } else {

@@ -17,7 +17,7 @@ import os
 
 # MSIG_LIBRARY selects another build of the library (e.g. a tuning variant)
 LIB_PATH = Path(os.environ.get("MSIG_LIBRARY") or Path(__file__).with_name("libmsig.so"))
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 MSIG_EINVAL = -22
 MSIG_ENOTSUP = -95
@@ -53,6 +53,8 @@ class MsigJob(ctypes.Structure):
         ("nfft", ctypes.c_int32),
         ("nseg", ctypes.c_int32),
         ("scale", ctypes.c_double),
+        ("lead", ctypes.c_int64),
+        ("nvalid", ctypes.c_int64),
     ]
 
 

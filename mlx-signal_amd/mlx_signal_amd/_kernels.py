@@ -81,16 +81,22 @@ def _rows(t: torch.Tensor) -> torch.Tensor:
     return t
 
 
-def _job(kind, x2, y2, nperseg, nstep, nfft, detrend, onesided, apply_factor, scale):
-    rows, n = x2.shape
+def _job(kind, x2, y2, nperseg, nstep, nfft, detrend, onesided, apply_factor, scale,
+         lead=0, n_ext=None):
+    """Job descriptor for rows x2 (and y2).  lead / n_ext: the segments are cut
+    from each row extended by `lead` zeros in front and zeros after it up to
+    n_ext samples in all (per-frame kinds; read in the kernel, not copied)."""
+    rows, stored = x2.shape
+    n = stored if n_ext is None else int(n_ext)
     cplx = x2.dtype == torch.complex64
     flags = (_hip.F_COMPLEX_INPUT if cplx else 0) | (_hip.F_ONESIDED if onesided else 0)
     flags |= _hip.F_APPLY_FACTOR if apply_factor else 0
     flags |= _hip.F_DETREND_CONST if _is_const(detrend) else 0
     nseg = (n - nperseg) // nstep + 1
-    ld_y = y2.stride(0) if y2 is not None else n
+    ld_y = y2.stride(0) if y2 is not None else stored
+    ext = lead != 0 or n != stored
     return _hip.MsigJob(kind, flags, rows, n, x2.stride(0), ld_y, nperseg, nstep, nfft, nseg,
-                        float(scale))
+                        float(scale), int(lead), stored if ext else 0)
 
 
 def _check_inputs(x2, y2, win, nperseg):
@@ -162,10 +168,12 @@ def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, oneside
 
 
 def spectral_frames(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, onesided=True,
-                    apply_factor=False, scale=1.0):
+                    apply_factor=False, scale=1.0, lead=0, n_ext=None):
     """Per-segment X ("complex", times sqrt(scale)), |X|^2 ("power"),
     conj(X) Y ("csd") or the trio ("coh"), times ``scale``.  Returns
-    (rows, nseg, bins) tensors."""
+    (rows, nseg, bins) tensors.  lead / n_ext: segments are cut from each row
+    extended by `lead` zeros in front and zeros after it to n_ext samples
+    (the kernel reads the zeros; nothing is copied)."""
     x2 = _rows(x2)
     y2 = None if y2 is None else _rows(y2)
     _check_inputs(x2, y2, win, nperseg)
@@ -173,7 +181,8 @@ def spectral_frames(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, onesi
     if code in (_hip.CSD_FRAMES, _hip.COH_FRAMES) and y2 is None:
         raise ValueError(f"{kind} frames need two signals")
     lib = _hip.load()
-    job = _job(code, x2, y2, nperseg, nstep, nfft, detrend, onesided, apply_factor, scale)
+    job = _job(code, x2, y2, nperseg, nstep, nfft, detrend, onesided, apply_factor, scale,
+               lead, n_ext)
     cplx = x2.dtype == torch.complex64
     bins = nfft // 2 + 1 if (onesided and not cplx) else nfft
     rows = x2.shape[0]

@@ -180,13 +180,23 @@ static int check_size(std::mt19937_64& rng) {
 
 
 // Welch mean of one row exactly as k_mean does it for a group of TPF threads.
-template <int N, bool HALF>
-static int check_stream(std::mt19937_64& rng, int nperseg, int hop, int n, int detrend) {
+// lead / tail > 0: the row is read through the virtual zero extension
+// (lead zeros in front of the n stored samples, tail zeros after them), as
+// the per-frame kernels read stft rows; NaN guards on both sides of the
+// stored samples poison any read outside them.
+template <int N, bool HALF, bool EXT = false>
+static int check_stream(std::mt19937_64& rng, int nperseg, int hop, int n, int detrend,
+                        int lead = 0, int tail = 0) {
   constexpr int E = N < 16 ? N : 16;
   constexpr int TPF = N / E;
   std::normal_distribution<double> nd;
-  std::vector<float> x(n + 2 * N, NAN);
+  const int guard = 2 * N;
+  std::vector<float> buf(guard + n + 2 * N, NAN);
+  float* x = buf.data() + guard;  // the stored row
   for (int i = 0; i < n; ++i) x[i] = (float)(nd(rng) + 0.5 * std::sin(0.3 * i) + 2.0);
+  const int n_ext = lead + n + tail;
+  std::vector<float> xe(n_ext, 0.f);  // the extended row, for the reference
+  for (int i = 0; i < n; ++i) xe[lead + i] = x[i];
   std::vector<cf> table(N);
   for (int i = 0; i < N; ++i) {
     const double ang = -2.0 * M_PI * i / N;
@@ -195,14 +205,16 @@ static int check_stream(std::mt19937_64& rng, int nperseg, int hop, int n, int d
   std::vector<double> wd(nperseg);
   for (int i = 0; i < nperseg; ++i) wd[i] = 0.5 - 0.5 * std::cos(2.0 * M_PI * i / nperseg);
   msig::KArgs a{};
-  a.nseg = (n - nperseg) / hop + 1;
+  a.nseg = (n_ext - nperseg) / hop + 1;
   a.nperseg = nperseg;
   a.hop = hop;
   a.units = (int32_t)((a.nseg + 1) / 2);
   a.detrend = detrend;
-  std::vector<msig::UnitStream<N, E, false, HALF>> st(TPF);
+  a.lead = lead;
+  a.nvalid = n;
+  std::vector<msig::UnitStream<N, E, false, HALF, EXT>> st(TPF);
   for (int t = 0; t < TPF; ++t) {
-    st[t].row = reinterpret_cast<const char*>(x.data());
+    st[t].row = reinterpret_cast<const char*>(x);
     st[t].t = t;
     st[t].fetch(0, true, a);
   }
@@ -241,9 +253,9 @@ static int check_stream(std::mt19937_64& rng, int nperseg, int hop, int n, int d
   for (int s = 0; s < a.nseg; ++s) {
     std::vector<cd> seg(N, 0.0);
     double m = 0;
-    for (int i = 0; i < nperseg; ++i) m += x[(size_t)s * hop + i];
+    for (int i = 0; i < nperseg; ++i) m += xe[(size_t)s * hop + i];
     m = detrend ? m / nperseg : 0.0;
-    for (int i = 0; i < nperseg; ++i) seg[i] = (x[(size_t)s * hop + i] - m) * wd[i];
+    for (int i = 0; i < nperseg; ++i) seg[i] = (xe[(size_t)s * hop + i] - m) * wd[i];
     auto X = dft(seg);
     for (int k = 0; k < N; ++k) P[k] += std::norm(X[k]);
   }
@@ -251,12 +263,12 @@ static int check_stream(std::mt19937_64& rng, int nperseg, int hop, int n, int d
   for (int k = 0; k < N; ++k) {
     const double got = 0.5 * (S[k] + S[(N - k) & (N - 1)]);
     peak = std::max(peak, P[k]);
-    err = std::max(err, std::abs(got - P[k]));
+    err = std::isfinite(got) ? std::max(err, std::abs(got - P[k])) : INFINITY;  // NaN poisons
   }
   const double rel = std::isfinite(err) ? err / peak : 1.0;
   const bool ok = rel < 1e-5;
-  std::printf("stream N=%5d %s nperseg=%d hop=%d nseg=%lld detrend=%d rel=%.3e %s\n", N,
-              HALF ? "half" : "gen ", nperseg, hop, (long long)a.nseg, detrend, rel,
+  std::printf("stream N=%5d %s nperseg=%d hop=%d nseg=%lld detrend=%d lead=%d tail=%d rel=%.3e %s\n",
+              N, HALF ? "half" : "gen ", nperseg, hop, (long long)a.nseg, detrend, lead, tail, rel,
               ok ? "ok" : "FAIL");
   return ok ? 0 : 1;
 }
@@ -293,6 +305,7 @@ static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detren
   a.units = (int32_t)((nseg + 1) / 2);
   a.run = run;
   a.detrend = detrend;
+  a.nvalid = n;
   const int groups = (a.units + run - 1) / run + 1;  // one group past the end
   std::vector<int> seen(a.units, 0);
   std::vector<double> S(N, 0.0);
@@ -388,7 +401,7 @@ static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detren
   for (int k = 0; k < N; ++k) {
     const double got = 0.5 * (S[k] + S[(N - k) & (N - 1)]);
     peak = std::max(peak, P[k]);
-    err = std::max(err, std::abs(got - P[k]));
+    err = std::isfinite(got) ? std::max(err, std::abs(got - P[k])) : INFINITY;  // NaN poisons
   }
   const double rel = std::isfinite(err) ? err / peak : 1.0;
   const bool ok = once && in_bounds && rel < 1e-5;
@@ -511,6 +524,14 @@ int main() {
   bad += check_stream<64, false>(rng, 50, 20, 333, 1);
   bad += check_stream<256, false>(rng, 256, 100, 5000, 0);
   bad += check_stream<256, false>(rng, 200, 200, 4000, 1);
+  // virtual zero extension (stft boundary='zeros' + padded): lead = nperseg / 2
+  bad += check_stream<64, true, true>(rng, 64, 32, 64 * 7 + 5, 0, 32, 32 + 27);
+  bad += check_stream<256, true, true>(rng, 256, 128, 3000, 1, 128, 128 + 72);
+  bad += check_stream<64, false, true>(rng, 64, 16, 1001, 0, 32, 32 + 7);
+  bad += check_stream<256, false, true>(rng, 200, 50, 2222, 1, 100, 100 + 28);
+  // the extended stream with no zeros (lead 0, every sample stored) is the plain one
+  bad += check_stream<256, true, true>(rng, 256, 128, 256 + 128 * 8 + 50, 1);
+  bad += check_stream<256, false, true>(rng, 256, 100, 5000, 0);
   // the peeled hop == N/2 group loop: odd / even nseg, runs that do and do
   // not divide the unit count, a run of one
   bad += check_half_groups<64>(rng, 15, 3, 1);

@@ -119,3 +119,33 @@ def test_autotune_rejects_bad_arguments_without_gpu():
     bad = (_hip.MsigTile * 1)(_hip.MsigTile(8, 1024, 4, 2))
     assert lib.msig_batch_autotune(jobs, 1, None, bad, 1, 3, ms, ctypes.byref(best), None) == _hip.MSIG_EINVAL
     assert best.value == -1
+
+
+def test_job_struct_matches_header():
+    """msig_job's fields, in order, are the ctypes structure's."""
+    text = (ROOT / "include" / "msig.h").read_text()
+    body = re.search(r"typedef struct msig_job \{(.*?)\} msig_job;", text, re.S).group(1)
+    fields = re.findall(r"^\s*(?:int32_t|int64_t|double)\s+(\w+);", body, re.M)
+    assert fields == [f for f, _ in _hip.MsigJob._fields_]
+
+
+def test_zero_extension_is_for_frame_kinds_only():
+    """A job with a virtual lead is refused for mean kinds and checked for
+    range before any device call."""
+    if not _hip.LIB_PATH.exists():
+        pytest.skip("libmsig.so not built")
+    lib = _hip.load()
+    sz = ctypes.c_size_t(0)
+    # stft-like: 1000 stored samples, 128 zeros in front, extended length 1000 + 256 + 72
+    n_ext = 1000 + 256 + 72
+    nseg = (n_ext - 256) // 128 + 1
+    job = _hip.MsigJob(_hip.PSD_MEAN, _hip.F_ONESIDED, 2, n_ext, 1000, 1000, 256, 128, 256, nseg,
+                       1.0, 128, 1000)
+    assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == _hip.MSIG_ENOTSUP
+    assert b"per-frame" in lib.msig_last_error()
+    job.kind = _hip.FRAMES_COMPLEX
+    assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == 0
+    job.nvalid = n_ext  # lead + stored past the extended row
+    assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == _hip.MSIG_EINVAL
+    job.nvalid, job.lead = 1000, -1
+    assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == _hip.MSIG_EINVAL

@@ -418,3 +418,52 @@ def test_kernel_timing_events(rng):
     assert n == 2 and tot > 0.0
     assert torch.equal(p, ref)
     assert _kernels.kernel_timing_read() == (0.0, 0)
+
+
+@pytest.mark.parametrize("nperseg,noverlap,cplx,detrend", [
+    (256, 128, False, False),        # hop == N/2 stream
+    (1024, 512, False, "constant"),
+    (256, 200, False, "constant"),   # general stream
+    (200, 150, False, False),        # nperseg < nfft
+    (256, 128, True, False),         # complex input
+])
+def test_frames_virtual_zero_extension_equals_padded(rng, nperseg, noverlap, cplx, detrend):
+    """The per-frame kernel reading stft's boundary zeros and padding in place
+    (lead / nvalid) gives the same bits as the kernel on the explicitly
+    zero-extended rows."""
+    n = 9000 + nperseg // 3
+    x = make_sig(rng, (3, n), complex_=cplx)
+    dev = torch.device("cuda")
+    xm = torch.from_numpy(x).to(dev)
+    nfft = 256 if nperseg == 200 else nperseg
+    nstep = nperseg - noverlap
+    lead = nperseg // 2
+    nadd = (-(n + 2 * lead - nperseg) % nstep) % nperseg
+    n_ext = n + 2 * lead + nadd
+    xp = torch.cat([torch.zeros(3, lead, dtype=xm.dtype, device=dev), xm,
+                    torch.zeros(3, lead + nadd, dtype=xm.dtype, device=dev)], -1)
+    import scipy.signal as sps
+
+    win = torch.from_numpy(sps.get_window("hann", nperseg).astype(np.float32)).to(dev)
+    onesided = not cplx
+    (a,) = _kernels.spectral_frames("complex", xm, None, win, nperseg, nstep, nfft, detrend,
+                                    onesided, False, 0.7, lead, n_ext)
+    (b,) = _kernels.spectral_frames("complex", xp, None, win, nperseg, nstep, nfft, detrend,
+                                    onesided, False, 0.7)
+    assert a.shape == b.shape and torch.equal(a, b)
+    if not cplx:  # only the complex frames (stft) read through the extension
+        with pytest.raises(Exception, match="complex per-frame"):
+            _kernels.spectral_frames("power", xm, None, win, nperseg, nstep, nfft, detrend,
+                                     True, True, 0.7, lead, n_ext)
+
+
+def test_stft_reads_boundary_zeros_in_place(rng):
+    """stft(boundary='zeros', padded=True) runs one fused per-frame launch on
+    the unextended rows (the kernel reads the zeros) and matches the oracle."""
+    x = make_sig(rng, (4, 20000))
+    xm = torch.from_numpy(x).cuda()
+    before = _kernels.launches["frames_complex"]
+    _, _, z = msig.stft(xm, nperseg=512)
+    assert _kernels.launches["frames_complex"] == before + 1
+    ref = oracle.stft(x, nperseg=512)[2]
+    assert_peak_close(z, ref, TOL)
