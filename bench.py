@@ -551,7 +551,7 @@ def main():
             w, e, _, _ = time_steps(torch, fn, xs16, max(5, args.steps // 2), 2)
             secondary[name] = {"ms_event": e, "ms_wall": w, "GBps": nbytes / (e * 1e-3) / 1e9,
                                "samples_per_s": 16 * NS / (e * 1e-3)}
-        # istft of the 16 x 2^20 stft (fused inverse + gather overlap-add)
+        # istft of the 16 x 2^20 stft (hop = N/2: inverse and overlap-add in one kernel)
         zs = [msig.stft(b, nperseg=1024)[2] for b in xs16]
         nseg_i = zs[0].shape[-1]
         w, e, _, _ = time_steps(torch, lambda z: msig.istft(z, nperseg=1024), zs,

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define MSIG_ABI_VERSION 4
+#define MSIG_ABI_VERSION 5
 
 #define MSIG_EINVAL (-22)
 #define MSIG_ENOTSUP (-95)
@@ -255,6 +255,20 @@ int msig_mean_workspace_bytes(int32_t kind, int64_t rows, int64_t n, int32_t npe
  * nperseg a power of two in [MSIG_MIN_NFFT, MSIG_MAX_NFFT], else MSIG_ENOTSUP. */
 int msig_istft_frames(const void* z, int64_t rows, int32_t nseg, int32_t nperseg,
                       const float* weff, float* frames, void* stream);
+
+/* istft with the overlap-add fused, for hop = nperseg/2 (reference: the two kernels of
+ * istft_frames, src/mlx_signal_processing/_stft_metal.py:753-815, called from istft,
+ * spectral.py:824-992): msig_istft_frames followed by msig_overlap_add(channels = 1,
+ * hop = nperseg/2) on its frames, without the [rows][nseg][nperseg] frames in memory: the
+ * same frame values summed in the same order and divided by the same window power (the
+ * inverse transform may differ in the last bit where a multiply-add fuses differently).  z complex64 [rows][nseg][nperseg/2+1];
+ * weff = window * scale / nperseg and win (the window, for the window-power
+ * divide) float32[nperseg]; out float32 [rows][count] = samples first .. first+count-1,
+ * first + count <= (nseg + 1) * nperseg/2.  nperseg a power of two in
+ * [MSIG_MIN_NFFT, MSIG_MAX_NFFT], else MSIG_ENOTSUP. */
+int msig_istft_ola(const void* z, int64_t rows, int32_t nseg, int32_t nperseg,
+                   const float* weff, const float* win, int64_t first, int64_t count, float* out,
+                   void* stream);
 
 /* Overlap-add with the window-power divide (reference: _stft_metal.py:793-815,
  * the gather kernel of istft_frames): frames float32 [rows][nseg][nperseg][channels]

@@ -532,6 +532,17 @@ static hipError_t dispatch_frames(int N, int ok, bool cplx, bool ext, const KArg
   return hipErrorInvalidValue;
 }
 
+static hipError_t dispatch_istft_ola(int N, const KArgs& a, int64_t rows, hipStream_t s) {
+  switch (N) {
+#define C(NV) \
+  case NV:    \
+    return launch_istft_ola_##NV(a, rows, s);
+    C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+  }
+  return hipErrorInvalidValue;
+}
+
 static hipError_t dispatch_istft(int N, const KArgs& a, int64_t rows, hipStream_t s) {
   switch (N) {
 #define C(NV) \
@@ -811,7 +822,7 @@ int msig_spectral_frames(const msig_job* job, const void* x, const void* y, cons
     return fail(MSIG_EINVAL, "coherence frames need three outputs");
   const bool cplx = job->flags & MSIG_F_COMPLEX_INPUT;
   const int N = job->nfft;
-  const Geo g = geometry(job->rows, job->nseg, N, cplx, FRAMES_MAX_RUN);
+  const Geo g = geometry(job->rows, job->nseg, N, cplx, FRAMES_MAX_RUN, FRAMES_TARGET_BLOCKS);
   const cf* tw = nullptr;
   rc = twiddles(N, &tw);
   if (rc) return rc;
@@ -1233,6 +1244,46 @@ int msig_istft_frames(const void* z, int64_t rows, int32_t nseg, int32_t nperseg
   a.nbins = N / 2 + 1;
   hipError_t e = dispatch_istft(N, a, rows, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "inverse frames kernel launch");
+  return 0;
+}
+
+int msig_istft_ola(const void* z, int64_t rows, int32_t nseg, int32_t nperseg,
+                   const float* weff, const float* win, int64_t first, int64_t count, float* out,
+                   void* stream) {
+  if (!z || !weff || !win || !out) return fail(MSIG_EINVAL, "NULL pointer");
+  if (rows < 1 || nseg < 1) return fail(MSIG_EINVAL, "rows and nseg must be >= 1");
+  if (!pow2(nperseg) || nperseg < MSIG_MIN_NFFT || nperseg > MSIG_MAX_NFFT)
+    return fail(MSIG_ENOTSUP, "nperseg %d outside the fused power-of-two range [%d, %d]",
+                nperseg, MSIG_MIN_NFFT, MSIG_MAX_NFFT);
+  const int64_t outlen = (int64_t)(nseg + 1) * (nperseg / 2);
+  if (first < 0 || count < 0 || first + count > outlen)
+    return fail(MSIG_EINVAL, "samples [%lld, %lld) outside the %lld-sample output",
+                (long long)first, (long long)(first + count), (long long)outlen);
+  if (count == 0) return 0;
+  const int N = nperseg;
+  const cf* tw = nullptr;
+  int rc = twiddles(N, &tw);
+  if (rc) return rc;
+  const Geo g = geometry(rows, nseg, N, false, ISTFT_OLA_MAX_RUN, ISTFT_OLA_TARGET_BLOCKS);
+  if (rows * (int64_t)g.bpr > 0x7fffffff) return fail(MSIG_EINVAL, "too many rows");
+  KArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.x = z;
+  a.y = win;
+  a.win = weff;
+  a.twt = tw;
+  a.out0 = out;
+  a.nseg = nseg;
+  a.nperseg = N;
+  a.hop = N / 2;
+  a.units = g.units;
+  a.run = g.run;
+  a.bpr = g.bpr;
+  a.nbins = N / 2 + 1;
+  a.ola_first = first;
+  a.ola_count = count;
+  hipError_t e = dispatch_istft_ola(N, a, rows, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "fused istft kernel launch");
   return 0;
 }
 

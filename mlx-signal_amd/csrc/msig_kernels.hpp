@@ -689,16 +689,51 @@ __global__ __launch_bounds__(BT, (frames_waves<N, OK>())) void k_frames(const KA
 // Hermitian extension X[k] = conj X[N - k] above N/2, the imaginary parts of
 // the DC and Nyquist bins ignored as irfft does) are packed as
 // Z = X_s + i X_{s+1}; ifft(Z) = conj(fft(conj Z)) / N has real part x_s and
-// imaginary part x_{s+1}.  The frames are written multiplied by weff
-// (= window * scale / N) as [rows][nseg][N] float32, for the overlap-add.
-// Counterpart of the inverse kernel of istft_frames (_stft_metal.py:753-791).
+// imaginary part x_{s+1}.  After the transform register e of thread t holds
+// sample bin(t, e) of both frames, before the weff (= window * scale / N)
+// factor.  Counterpart of the inverse kernel of istft_frames
+// (_stft_metal.py:753-791).
+template <int N, int E, bool TWL>
+__device__ __forceinline__ void inverse_unit(const MSIG_GLOBAL cf* zrow, int nb, int64_t s0,
+                                             bool active, bool two, int t, const cf* tw,
+                                             cf* lds, cf (&z)[E]) {
+  using PL = Plan<N, E>;
+  constexpr int TPF = PL::TPF;
+  constexpr int M = N / 2;
+  // X[k] of one frame, k < N, from its one-sided bins
+  auto bin_of = [&](const MSIG_GLOBAL cf* f, int k) -> cf {
+    if (k == 0) return cf{f[0].re, 0.f};
+    if (k == M) return cf{f[M].re, 0.f};
+    if (k < M) return cf{f[k].re, f[k].im};
+    const cf c = cf{f[N - k].re, f[N - k].im};
+    return cf{c.re, -c.im};
+  };
+  const MSIG_GLOBAL cf* f0 = zrow + s0 * nb;
+  const MSIG_GLOBAL cf* f1 = f0 + nb;
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + TPF * e;
+      const cf xs = bin_of(f0, k);
+      const cf xt = two ? bin_of(f1, k) : cf{0.f, 0.f};
+      // conj(X_s + i X_t)
+      z[e] = cf{xs.re - xt.im, -(xs.im + xt.re)};
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) z[e] = cf{0.f, 0.f};
+  }
+  fft_passes<N, E, TWL>(t, z, tw, lds);
+}
+
+// Frames of istft: x_s = Re F * weff, x_{s+1} = -Im F * weff written as
+// [rows][nseg][N] float32 for the overlap-add.
 template <int N, int E>
 __device__ __forceinline__ void istft_body(const KArgs& a, int64_t row, int bir, cf* lds_all,
                                            const cf* twl) {
   using PL = Plan<N, E>;
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
-  constexpr int M = N / 2;
   const int tid = threadIdx.x;
   const int g = tid / TPF, t = tid % TPF;
   cf* lds = lds_all + g * PL::LDS;
@@ -720,15 +755,6 @@ __device__ __forceinline__ void istft_body(const KArgs& a, int64_t row, int bir,
   const MSIG_GLOBAL cf* zrow = as_global(static_cast<const cf*>(a.x)) + row * a.nseg * nb;
   MSIG_GLOBAL float* orow = as_global_mut(static_cast<float*>(a.out0)) + row * a.nseg * N;
 
-  // X[k] of one frame, k < N, from its one-sided bins
-  auto bin_of = [&](const MSIG_GLOBAL cf* f, int k) -> cf {
-    if (k == 0) return cf{f[0].re, 0.f};
-    if (k == M) return cf{f[M].re, 0.f};
-    if (k < M) return cf{f[k].re, f[k].im};
-    const cf c = cf{f[N - k].re, f[N - k].im};
-    return cf{c.re, -c.im};
-  };
-
   // Every group runs a.run units (those past the row's end transform zeros
   // and write nothing), so the groups of a block run the same barriers.
   for (int it = 0; it < a.run; ++it) {
@@ -736,23 +762,8 @@ __device__ __forceinline__ void istft_body(const KArgs& a, int64_t row, int bir,
     const bool active = unit < a.units;  // group-uniform
     const int64_t s0 = 2 * (int64_t)unit;
     const bool two = active && s0 + 1 < a.nseg;
-    const MSIG_GLOBAL cf* f0 = zrow + s0 * nb;
-    const MSIG_GLOBAL cf* f1 = f0 + nb;
     cf z[E];  // FFT input position t + TPF e
-    if (active) {
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int k = t + TPF * e;
-        const cf xs = bin_of(f0, k);
-        const cf xt = two ? bin_of(f1, k) : cf{0.f, 0.f};
-        // conj(X_s + i X_t)
-        z[e] = cf{xs.re - xt.im, -(xs.im + xt.re)};
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < E; ++e) z[e] = cf{0.f, 0.f};
-    }
-    fft_passes<N, E, TH::LDS>(t, z, tw, lds);
+    inverse_unit<N, E, TH::LDS>(zrow, nb, s0, active, two, t, tw, lds, z);
     // sample n = bin(t, e): x_s = Re F / N, x_t = -Im F / N (1/N folded in w)
     if (active) {
 #pragma unroll
@@ -760,6 +771,125 @@ __device__ __forceinline__ void istft_body(const KArgs& a, int64_t row, int bir,
         const int n = PL::bin(t, e);
         orow[s0 * N + n] = z[e].re * w[e];
         if (two) orow[(s0 + 1) * N + n] = -z[e].im * w[e];
+      }
+    }
+    group_sync<TPF>();
+  }
+}
+
+// istft with the overlap-add fused, for hop = N/2.  Frame s covers
+// half-blocks s and s+1 of the output (M = N/2 samples each), so half-block
+// j is frame j-1's second half plus frame j's first half.  A group walks
+// units u (frames 2u, 2u+1) in order: it writes half-blocks 2u (with the
+// second half of frame 2u-1 carried in registers from the previous unit)
+// and 2u+1, and carries frame 2u+1's second half; the row's last frame also
+// writes the final half-block.  A group first transforms the unit before
+// its own for that carry (every group does, on zeros at a row's start, so
+// the groups of a block run the same barriers).  Register e of thread t
+// holds sample t + TPF k with k = (e / R) + (E / R) (e % R) (R the last
+// pass's radix), so a thread's first-half samples are k < E/2 and sample
+// r + M is k + E/2 of the same thread.
+//
+// Each output sample i = j M + r is computed as the gather overlap-add
+// (k_overlap_add) computes it from the frames of istft_body: the frame
+// values x * weff rounded, summed from 0 in increasing frame order, norm =
+// sum of win[i - s hop]^2 by fmaf in the same order, divided where
+// norm > 1e-10 (the transform itself may round differently in the last bit,
+// where the compiler fuses a multiply-add differently).  Written are samples first <= i < first + count, as
+// out[row][i - first].  KArgs: x = spectra, win = weff, y = window,
+// out0 = output, ola_first / ola_count.
+template <int N, int E>
+__device__ __forceinline__ void istft_ola_body(const KArgs& a, int64_t row, int bir, cf* lds_all,
+                                               const cf* twl) {
+#pragma clang fp contract(off)
+  using PL = Plan<N, E>;
+  constexpr int TPF = PL::TPF;
+  constexpr int GPB = BT / TPF;
+  constexpr int M = N / 2;
+  constexpr int H = E / 2;
+  constexpr int R = PL::radix(PL::NP - 1);
+  constexpr int ER = E / R;
+  const int tid = threadIdx.x;
+  const int g = tid / TPF, t = tid % TPF;
+  cf* lds = lds_all + g * PL::LDS;
+  const int unit0 = (bir * GPB + g) * a.run;
+
+  using TH = TwiddleHome<N, E, 0, frames_twl<N>()>;
+  cf twr[TH::REGS];
+  const cf* tw;
+  if constexpr (TH::LDS) {
+    tw = twl;
+  } else {
+    load_twiddles<N, E>(t, as_global(a.twt), twr);
+    tw = twr;
+  }
+  // register of sample t + TPF k
+  auto reg = [](int k) { return (k % ER) * R + k / ER; };
+  // weff and the window are read where used (cache hits): kept in registers
+  // they pushed the larger sizes to one wave per SIMD
+  const MSIG_GLOBAL float* wf = as_global(a.win) + t;
+  const MSIG_GLOBAL float* wn = as_global(static_cast<const float*>(a.y)) + t;
+  const int nb = a.nbins;
+  const MSIG_GLOBAL cf* zrow = as_global(static_cast<const cf*>(a.x)) + row * a.nseg * nb;
+  MSIG_GLOBAL float* orow = as_global_mut(static_cast<float*>(a.out0)) + row * a.ola_count;
+  const int64_t first = a.ola_first, last = a.ola_first + a.ola_count;
+
+  // half-block j: sum of lo (frame j, first half; present when lo_ok) and
+  // hi (frame j-1, second half; present when hi_ok) at sample r = t + TPF k
+  auto write_half = [&](int64_t j, const float* hi, bool hi_ok, const float* lo, bool lo_ok) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const int64_t i = j * M + t + TPF * k;
+      float acc = 0.f, nrm = 0.f;
+      if (hi_ok) {
+        const float v = wn[M + TPF * k];  // frame j-1 holds r at offset M + r
+        acc += hi[k];
+        nrm = fmaf(v, v, nrm);
+      }
+      if (lo_ok) {
+        const float v = wn[TPF * k];
+        acc += lo[k];
+        nrm = fmaf(v, v, nrm);
+      }
+      if (i >= first && i < last) orow[i - first] = acc / (nrm > 1e-10f ? nrm : 1.f);
+    }
+  };
+
+  float carry[H];  // frame 2u-1's second half (x * weff) at r = t + TPF k
+  {
+    // lead-in: the unit before this group's first one, for the carry
+    const int unit = unit0 - 1;
+    const bool active = unit >= 0 && unit < a.units;
+    const int64_t s0 = 2 * (int64_t)unit;
+    const bool two = active && s0 + 1 < a.nseg;
+    cf z[E];
+    inverse_unit<N, E, TH::LDS>(zrow, nb, s0, active, two, t, tw, lds, z);
+#pragma unroll
+    for (int k = 0; k < H; ++k) carry[k] = -z[reg(k + H)].im * wf[TPF * (k + H)];
+    group_sync<TPF>();
+  }
+  for (int it = 0; it < a.run; ++it) {
+    const int unit = unit0 + it;
+    const bool active = unit < a.units;  // group-uniform
+    const int64_t s0 = 2 * (int64_t)unit;
+    const bool two = active && s0 + 1 < a.nseg;
+    cf z[E];
+    inverse_unit<N, E, TH::LDS>(zrow, nb, s0, active, two, t, tw, lds, z);
+    if (active) {
+      float xs_lo[H], xs_hi[H], xt_lo[H];
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        const float wl = wf[TPF * k], wh = wf[TPF * (k + H)];
+        xs_lo[k] = z[reg(k)].re * wl;
+        xs_hi[k] = z[reg(k + H)].re * wh;
+        xt_lo[k] = -z[reg(k)].im * wl;
+      }
+      write_half(s0, carry, s0 >= 1, xs_lo, true);
+      write_half(s0 + 1, xs_hi, true, xt_lo, two);
+      if (two) {
+#pragma unroll
+        for (int k = 0; k < H; ++k) carry[k] = -z[reg(k + H)].im * wf[TPF * (k + H)];
+        if (unit == a.units - 1) write_half(s0 + 2, carry, true, carry, false);
       }
     }
     group_sync<TPF>();
@@ -777,6 +907,19 @@ __global__ __launch_bounds__(BT) void k_istft(const KArgs a) {
     __syncthreads();
   }
   istft_body<N, E>(a, blockIdx.x / a.bpr, blockIdx.x % a.bpr, lds_all, twl);
+}
+
+template <int N, int E>
+__global__ __launch_bounds__(BT, (N >= 4096 ? 1 : 2)) void k_istft_ola(const KArgs a) {
+  using PL = Plan<N, E>;
+  using TH = TwiddleHome<N, E, 0, frames_twl<N>()>;
+  __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
+  __shared__ cf twl[TH::LDS_ELEMS];
+  if constexpr (TH::LDS) {
+    fill_twiddle_lds<N, E>(twl, a.twt);
+    __syncthreads();
+  }
+  istft_ola_body<N, E>(a, blockIdx.x / a.bpr, blockIdx.x % a.bpr, lds_all, twl);
 }
 
 // One entry of a ragged batch's work queue: block `blk` (row * bpr + bir) of

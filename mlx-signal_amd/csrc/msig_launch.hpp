@@ -39,6 +39,13 @@ inline Geo geometry(int64_t rows, int64_t nseg, int N, bool cplx, int maxrun,
 
 constexpr int MEAN_MAX_RUN = 32;    // <= 64 segments summed sequentially in FP32
 constexpr int FRAMES_MAX_RUN = 8;
+// Per-frame kernels: about 1536 blocks (stft 16 x 2^20, nperseg 1024, ran in
+// 52 us at this target against 59 us at 1024; the power frames were flat).
+constexpr int64_t FRAMES_TARGET_BLOCKS = 1536;
+// Fused istft (hop = N/2): each group transforms one unit ahead of its own
+// for the overlap carry, so longer runs waste less.
+constexpr int ISTFT_OLA_MAX_RUN = 16;
+constexpr int64_t ISTFT_OLA_TARGET_BLOCKS = 512;
 
 // Split-precision matrix-core form (nfft = nperseg = 256, welch mean): one
 // wave per segment pair, so a 256-thread block carries 4 groups.
@@ -70,7 +77,8 @@ hipError_t launch_mean_mfma256(int prec, const KArgs& a, int64_t rows, hipStream
   hipError_t launch_mean_batch_##NV(int kind, bool cplx, bool half, int sched,             \
                                     const KArgs* jobs, const WorkItem* queue, int32_t nwork, \
                                     float* ws, int grid, hipStream_t s);                    \
-  hipError_t launch_istft_##NV(const KArgs& a, int64_t rows, hipStream_t s);
+  hipError_t launch_istft_##NV(const KArgs& a, int64_t rows, hipStream_t s);                 \
+  hipError_t launch_istft_ola_##NV(const KArgs& a, int64_t rows, hipStream_t s);
 MSIG_DECLARE_LAUNCHERS(16)
 MSIG_DECLARE_LAUNCHERS(32)
 MSIG_DECLARE_LAUNCHERS(64)
@@ -160,6 +168,11 @@ MSIG_DECLARE_LAUNCHERS(4096)
   hipError_t launch_istft_##NV(const KArgs& a, int64_t rows, hipStream_t s) {              \
     const dim3 grid((unsigned)(rows * a.bpr)), block(BT);                                  \
     hipLaunchKernelGGL((k_istft<NV, (NV < 16 ? NV : 16)>), grid, block, 0, s, a);          \
+    return hipGetLastError();                                                              \
+  }                                                                                        \
+  hipError_t launch_istft_ola_##NV(const KArgs& a, int64_t rows, hipStream_t s) {          \
+    const dim3 grid((unsigned)(rows * a.bpr)), block(BT);                                  \
+    hipLaunchKernelGGL((k_istft_ola<NV, (NV < 16 ? NV : 16)>), grid, block, 0, s, a);      \
     return hipGetLastError();                                                              \
   }                                                                                        \
   }

@@ -28,6 +28,7 @@ struct KArgs {
   int64_t ws_off;    // batch plans: this job's partials start at ws_base + ws_off
   int64_t lead;      // zero samples virtually in front of each row (frames kinds)
   int64_t nvalid;    // samples stored per row; the extended row reads zero past them
+  int64_t ola_first, ola_count;  // fused istft: output samples [first, first + count)
 };
 
 // Raw samples of consecutive units of one row, fetched one unit ahead so the

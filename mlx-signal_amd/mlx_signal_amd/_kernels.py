@@ -379,11 +379,34 @@ def overlap_add(frames, win, nperseg: int, hop: int, first: int, count: int):
     ch = 2 if cplx else 1
     out = torch.empty((rows, count, 2) if cplx else (rows, count), dtype=torch.float32,
                       device=frames.device)
+    if out.numel() == 0:
+        return torch.view_as_complex(out) if cplx else out
     rc = _hip.load().msig_overlap_add(_hip.ptr(fr), rows, nseg, nperseg, hop, ch, _hip.ptr(win),
                                      first, count, _hip.ptr(out), _hip.stream_of(fr))
     _hip.check(rc, "overlap_add")
     launches["overlap_add"] += 1
     return torch.view_as_complex(out) if cplx else out
+
+
+def istft_ola(z3, weff, win, nperseg: int, first: int, count: int):
+    """istft for hop = nperseg/2 with the overlap-add fused: the result of
+    ``overlap_add(istft_frames(z3, weff, nperseg), win, nperseg, nperseg // 2,
+    first, count)`` (the same sums in the same order) without the frames in
+    memory (reference: the two kernels of istft_frames, _stft_metal.py:753-815).  z3: (rows, nseg,
+    nperseg/2 + 1) complex64; returns (rows, count) float32."""
+    if z3.dtype != torch.complex64 or z3.dim() != 3 or z3.shape[-1] != nperseg // 2 + 1:
+        raise ValueError("istft_ola takes (rows, nseg, nperseg/2+1) complex64")
+    z3 = z3.contiguous()
+    rows, nseg, _ = z3.shape
+    out = torch.empty(rows, count, dtype=torch.float32, device=z3.device)
+    if out.numel() == 0:
+        return out
+    rc = _hip.load().msig_istft_ola(_hip.ptr(z3), rows, nseg, nperseg, _hip.ptr(weff),
+                                   _hip.ptr(win), first, count, _hip.ptr(out),
+                                   _hip.stream_of(z3))
+    _hip.check(rc, "istft_ola")
+    launches["istft_ola"] += 1
+    return out
 
 
 MEDIAN_MAX_NSEG = 16000

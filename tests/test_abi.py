@@ -149,3 +149,21 @@ def test_zero_extension_is_for_frame_kinds_only():
     assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == _hip.MSIG_EINVAL
     job.nvalid, job.lead = 1000, -1
     assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == _hip.MSIG_EINVAL
+
+
+def test_fused_istft_rejects_bad_arguments_without_gpu():
+    """msig_istft_ola validates before any device call (fake pointers are
+    never dereferenced on these paths)."""
+    if not _hip.LIB_PATH.exists():
+        pytest.skip("libmsig.so not built")
+    lib = _hip.load()
+    p = ctypes.c_void_p(16)
+    assert lib.msig_istft_ola(None, 1, 4, 256, p, p, 0, 10, p, None) == _hip.MSIG_EINVAL
+    assert lib.msig_istft_ola(p, 0, 4, 256, p, p, 0, 10, p, None) == _hip.MSIG_EINVAL
+    assert lib.msig_istft_ola(p, 1, 4, 100, p, p, 0, 10, p, None) == _hip.MSIG_ENOTSUP
+    assert lib.msig_istft_ola(p, 1, 4, 8192, p, p, 0, 10, p, None) == _hip.MSIG_ENOTSUP
+    # the output spans (nseg + 1) * nperseg / 2 = 640 samples
+    assert lib.msig_istft_ola(p, 1, 4, 256, p, p, 1, 640, p, None) == _hip.MSIG_EINVAL
+    assert b"outside" in lib.msig_last_error()
+    assert lib.msig_istft_ola(p, 1, 4, 256, p, p, -1, 10, p, None) == _hip.MSIG_EINVAL
+    assert lib.msig_istft_ola(p, 1, 4, 256, p, p, 640, 0, p, None) == 0  # nothing to do

@@ -61,16 +61,76 @@ def test_istft_fused_matches_oracle(rng, nperseg, noverlap_frac, boundary):
                    boundary="zeros" if boundary else None, padded=boundary)
     _, z = _spectra(rng, n, (2,), **stft_kw)
     kw = dict(nperseg=nperseg, noverlap=noverlap, boundary=boundary)
-    before = _kernels.launches["istft_frames"]
+    fused = "istft_ola" if 2 * noverlap == nperseg else "istft_frames"  # hop = N/2: OLA fused
+    before = _kernels.launches[fused]
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")  # no boundary: hann's zero first sample fails NOLA
         t, x = msig.istft(z, **kw)
         t_ref, x_ref = oracle.istft(z, **kw)
-    assert _kernels.launches["istft_frames"] == before + 1
+    assert _kernels.launches[fused] == before + 1
     assert x.dtype == torch.float32 and x.device.type == "cuda"
     assert tuple(x.shape) == x_ref.shape
     assert_istft_close(x, x_ref, _norm("hann", nperseg, nperseg - noverlap, z.shape[-1], boundary))
     np.testing.assert_allclose(t.cpu().numpy(), t_ref, rtol=1e-6)
+
+
+def _ola_cases():
+    # (nperseg, rows, nseg): one and two frames, odd and even counts, and
+    # rows long enough for several units per group and several blocks per
+    # row (the overlap carry inside a group and the lead-in across groups)
+    return [(16, 2, 1), (16, 1, 2), (16, 3, 1001), (64, 1, 3), (64, 2, 40001),
+            (256, 3, 301), (256, 1, 4000), (1024, 1, 5001), (1024, 2, 64),
+            (2048, 2, 2049), (4096, 1, 301), (4096, 3, 2)]
+
+
+@pytest.mark.parametrize("nperseg,rows,nseg", _ola_cases())
+def test_fused_istft_equals_frames_then_overlap_add(rng, nperseg, rows, nseg):
+    """The fused hop = N/2 kernel gives the two-kernel result, for the whole
+    output, the boundary-trimmed span and a random span: the same frame
+    values summed in the same order and divided by the same window power,
+    so the only difference is the inverse transform's own rounding (the
+    compiler may fuse a multiply-add differently in the two kernels):
+    bound 1e-6 x the peak |x|."""
+    dev = torch.device("cuda")
+    m = nperseg // 2
+    z = torch.from_numpy((rng.standard_normal((rows, nseg, m + 1))
+                          + 1j * rng.standard_normal((rows, nseg, m + 1))).astype(np.complex64)).to(dev)
+    win_np = sps.get_window("hann", nperseg).astype(np.float32)
+    win = torch.from_numpy(win_np).to(dev)
+    weff = torch.from_numpy((win_np * (win_np.sum() / nperseg)).astype(np.float32)).to(dev)
+    frames = _kernels.istft_frames(z, weff, nperseg)
+    outlen = (nseg + 1) * m
+    a = int(rng.integers(0, outlen))
+    spans = [(0, outlen), (m, max(0, outlen - 2 * m)), (a, int(rng.integers(0, outlen - a + 1)))]
+    for first, count in spans:
+        want = _kernels.overlap_add(frames, win, nperseg, m, first, count)
+        got = _kernels.istft_ola(z, weff, win, nperseg, first, count)
+        assert got.shape == want.shape
+        if count:
+            d = float((got - want).abs().max() / want.abs().max().clamp_min(1e-30))
+            assert d <= 1e-6, (first, count, d)
+
+
+def test_istft_routes_half_overlap_to_fused_kernel(rng):
+    _, z = _spectra(rng, 1 << 14, (2,), nperseg=256)
+    before = dict(_kernels.launches)
+    _, x = msig.istft(z, nperseg=256)
+    assert _kernels.launches["istft_ola"] == before.get("istft_ola", 0) + 1
+    assert _kernels.launches["overlap_add"] == before.get("overlap_add", 0)
+    _, x_ref = oracle.istft(z, nperseg=256)
+    assert_istft_close(x, x_ref, _norm("hann", 256, 128, z.shape[-1], True))
+    msig.istft(z, nperseg=256, noverlap=192)  # other hops keep the two kernels
+    assert _kernels.launches["overlap_add"] == before.get("overlap_add", 0) + 1
+
+
+def test_istft_single_frame_with_boundary_is_empty(rng):
+    """One frame with boundary trimming leaves nperseg - 2 (nperseg // 2) = 0
+    samples (scipy returns an empty signal)."""
+    z = np.ones((129, 1), np.complex64)
+    for kw in (dict(nperseg=256), dict(nperseg=256, noverlap=192)):
+        _, x = msig.istft(z, **kw)
+        _, x_ref = oracle.istft(z, **kw)
+        assert tuple(x.shape) == x_ref.shape == (0,)
 
 
 def test_istft_no_overlap_boxcar(rng):
