@@ -259,9 +259,9 @@ int msig_istft_frames(const void* z, int64_t rows, int32_t nseg, int32_t nperseg
 /* istft with the overlap-add fused, for hop = nperseg/2 (reference: the two kernels of
  * istft_frames, src/mlx_signal_processing/_stft_metal.py:753-815, called from istft,
  * spectral.py:824-992): msig_istft_frames followed by msig_overlap_add(channels = 1,
- * hop = nperseg/2) on its frames, without the [rows][nseg][nperseg] frames in memory: the
- * same frame values summed in the same order and divided by the same window power (the
- * inverse transform may differ in the last bit where a multiply-add fuses differently).  z complex64 [rows][nseg][nperseg/2+1];
+ * hop = nperseg/2) on its frames, without the [rows][nseg][nperseg] frames in memory.  The
+ * window-power divide is folded into per-sample weights (weff / norm, rounded once), so
+ * results differ from the two-call form by a few ulps.  z complex64 [rows][nseg][nperseg/2+1];
  * weff = window * scale / nperseg and win (the window, for the window-power
  * divide) float32[nperseg]; out float32 [rows][count] = samples first .. first+count-1,
  * first + count <= (nseg + 1) * nperseg/2.  nperseg a power of two in

@@ -700,22 +700,25 @@ __device__ __forceinline__ void inverse_unit(const MSIG_GLOBAL cf* zrow, int nb,
   using PL = Plan<N, E>;
   constexpr int TPF = PL::TPF;
   constexpr int M = N / 2;
-  // X[k] of one frame, k < N, from its one-sided bins
-  auto bin_of = [&](const MSIG_GLOBAL cf* f, int k) -> cf {
-    if (k == 0) return cf{f[0].re, 0.f};
-    if (k == M) return cf{f[M].re, 0.f};
-    if (k < M) return cf{f[k].re, f[k].im};
+  // X[k] of one frame at k = t + TPF e: registers e < E/2 hold k < M (read
+  // directly), the others k >= M (the conjugate of bin N - k); the
+  // imaginary parts of DC (t = 0, e = 0) and Nyquist (t = 0, e = E/2) drop
+  auto bin_of = [&](const MSIG_GLOBAL cf* f, int e) -> cf {
+    const int k = t + TPF * e;
+    if (e < E / 2) {
+      const cf c = cf{f[k].re, f[k].im};
+      return cf{c.re, (e == 0 && t == 0) ? 0.f : c.im};
+    }
     const cf c = cf{f[N - k].re, f[N - k].im};
-    return cf{c.re, -c.im};
+    return cf{c.re, (e == E / 2 && t == 0) ? 0.f : -c.im};
   };
   const MSIG_GLOBAL cf* f0 = zrow + s0 * nb;
   const MSIG_GLOBAL cf* f1 = f0 + nb;
   if (active) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int k = t + TPF * e;
-      const cf xs = bin_of(f0, k);
-      const cf xt = two ? bin_of(f1, k) : cf{0.f, 0.f};
+      const cf xs = bin_of(f0, e);
+      const cf xt = two ? bin_of(f1, e) : cf{0.f, 0.f};
       // conj(X_s + i X_t)
       z[e] = cf{xs.re - xt.im, -(xs.im + xt.re)};
     }
@@ -790,18 +793,19 @@ __device__ __forceinline__ void istft_body(const KArgs& a, int64_t row, int bir,
 // pass's radix), so a thread's first-half samples are k < E/2 and sample
 // r + M is k + E/2 of the same thread.
 //
-// Each output sample i = j M + r is computed as the gather overlap-add
-// (k_overlap_add) computes it from the frames of istft_body: the frame
-// values x * weff rounded, summed from 0 in increasing frame order, norm =
-// sum of win[i - s hop]^2 by fmaf in the same order, divided where
-// norm > 1e-10 (the transform itself may round differently in the last bit,
-// where the compiler fuses a multiply-add differently).  Written are samples first <= i < first + count, as
-// out[row][i - first].  KArgs: x = spectra, win = weff, y = window,
-// out0 = output, ola_first / ola_count.
+// The overlap-add's divide by the window power (k_overlap_add: norm(i) =
+// sum over the covering frames of win^2, taken where it exceeds 1e-10) is
+// folded into the per-sample weights once per thread: a half-block covered
+// by two frames has norm fma(win[r], win[r], fma(win[M + r], win[M + r], 0))
+// at every j, so sample n of a frame weighs weff[n] / norm(n mod M), and
+// the output is one multiply and one FMA of the two transforms' outputs.
+// The row's two end half-blocks (one frame each) divide by their own norm.
+// Written are samples first <= i < first + count, as out[row][i - first].
+// KArgs: x = spectra, win = weff, y = window, out0 = output,
+// ola_first / ola_count.
 template <int N, int E>
 __device__ __forceinline__ void istft_ola_body(const KArgs& a, int64_t row, int bir, cf* lds_all,
                                                const cf* twl) {
-#pragma clang fp contract(off)
   using PL = Plan<N, E>;
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
@@ -825,37 +829,33 @@ __device__ __forceinline__ void istft_ola_body(const KArgs& a, int64_t row, int 
   }
   // register of sample t + TPF k
   auto reg = [](int k) { return (k % ER) * R + k / ER; };
-  // weff and the window are read where used (cache hits): kept in registers
-  // they pushed the larger sizes to one wave per SIMD
   const MSIG_GLOBAL float* wf = as_global(a.win) + t;
   const MSIG_GLOBAL float* wn = as_global(static_cast<const float*>(a.y)) + t;
+  float wi[E];  // weight of sample t + TPF k in a half-block covered by two frames
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const float lo = wn[TPF * k], hi = wn[M + TPF * k];
+    const float nrm = fmaf(lo, lo, fmaf(hi, hi, 0.f));
+    const float d = nrm > 1e-10f ? nrm : 1.f;
+    wi[k] = wf[TPF * k] / d;
+    wi[k + H] = wf[M + TPF * k] / d;
+  }
+  // weight of sample t + TPF k in an end half-block (one frame)
+  auto w_end = [&](int k) {
+    const float v = wn[TPF * k];
+    const float nrm = fmaf(v, v, 0.f);
+    return wf[TPF * k] / (nrm > 1e-10f ? nrm : 1.f);
+  };
   const int nb = a.nbins;
   const MSIG_GLOBAL cf* zrow = as_global(static_cast<const cf*>(a.x)) + row * a.nseg * nb;
   MSIG_GLOBAL float* orow = as_global_mut(static_cast<float*>(a.out0)) + row * a.ola_count;
   const int64_t first = a.ola_first, last = a.ola_first + a.ola_count;
-
-  // half-block j: sum of lo (frame j, first half; present when lo_ok) and
-  // hi (frame j-1, second half; present when hi_ok) at sample r = t + TPF k
-  auto write_half = [&](int64_t j, const float* hi, bool hi_ok, const float* lo, bool lo_ok) {
-#pragma unroll
-    for (int k = 0; k < H; ++k) {
-      const int64_t i = j * M + t + TPF * k;
-      float acc = 0.f, nrm = 0.f;
-      if (hi_ok) {
-        const float v = wn[M + TPF * k];  // frame j-1 holds r at offset M + r
-        acc += hi[k];
-        nrm = fmaf(v, v, nrm);
-      }
-      if (lo_ok) {
-        const float v = wn[TPF * k];
-        acc += lo[k];
-        nrm = fmaf(v, v, nrm);
-      }
-      if (i >= first && i < last) orow[i - first] = acc / (nrm > 1e-10f ? nrm : 1.f);
-    }
+  auto put = [&](int64_t j, int k, float v) {
+    const int64_t i = j * M + t + TPF * k;
+    if (i >= first && i < last) orow[i - first] = v;
   };
 
-  float carry[H];  // frame 2u-1's second half (x * weff) at r = t + TPF k
+  float carry[H];  // frame 2u-1's second half (transform output) at r = t + TPF k
   {
     // lead-in: the unit before this group's first one, for the carry
     const int unit = unit0 - 1;
@@ -865,7 +865,7 @@ __device__ __forceinline__ void istft_ola_body(const KArgs& a, int64_t row, int 
     cf z[E];
     inverse_unit<N, E, TH::LDS>(zrow, nb, s0, active, two, t, tw, lds, z);
 #pragma unroll
-    for (int k = 0; k < H; ++k) carry[k] = -z[reg(k + H)].im * wf[TPF * (k + H)];
+    for (int k = 0; k < H; ++k) carry[k] = -z[reg(k + H)].im;
     group_sync<TPF>();
   }
   for (int it = 0; it < a.run; ++it) {
@@ -876,20 +876,28 @@ __device__ __forceinline__ void istft_ola_body(const KArgs& a, int64_t row, int 
     cf z[E];
     inverse_unit<N, E, TH::LDS>(zrow, nb, s0, active, two, t, tw, lds, z);
     if (active) {
-      float xs_lo[H], xs_hi[H], xt_lo[H];
+      // x_s = Re F and x_{s+1} = -Im F (1/N and the window in the weights)
+      if (s0 >= 1) {
 #pragma unroll
-      for (int k = 0; k < H; ++k) {
-        const float wl = wf[TPF * k], wh = wf[TPF * (k + H)];
-        xs_lo[k] = z[reg(k)].re * wl;
-        xs_hi[k] = z[reg(k + H)].re * wh;
-        xt_lo[k] = -z[reg(k)].im * wl;
+        for (int k = 0; k < H; ++k)
+          put(s0, k, fmaf(carry[k], wi[k + H], z[reg(k)].re * wi[k]));
+      } else {
+#pragma unroll
+        for (int k = 0; k < H; ++k) put(s0, k, z[reg(k)].re * w_end(k));
       }
-      write_half(s0, carry, s0 >= 1, xs_lo, true);
-      write_half(s0 + 1, xs_hi, true, xt_lo, two);
       if (two) {
 #pragma unroll
-        for (int k = 0; k < H; ++k) carry[k] = -z[reg(k + H)].im * wf[TPF * (k + H)];
-        if (unit == a.units - 1) write_half(s0 + 2, carry, true, carry, false);
+        for (int k = 0; k < H; ++k)
+          put(s0 + 1, k, fmaf(z[reg(k + H)].re, wi[k + H], -z[reg(k)].im * wi[k]));
+#pragma unroll
+        for (int k = 0; k < H; ++k) carry[k] = -z[reg(k + H)].im;
+        if (unit == a.units - 1) {
+#pragma unroll
+          for (int k = 0; k < H; ++k) put(s0 + 2, k, carry[k] * w_end(k + H));
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < H; ++k) put(s0 + 1, k, z[reg(k + H)].re * w_end(k + H));
       }
     }
     group_sync<TPF>();

@@ -391,8 +391,8 @@ def overlap_add(frames, win, nperseg: int, hop: int, first: int, count: int):
 def istft_ola(z3, weff, win, nperseg: int, first: int, count: int):
     """istft for hop = nperseg/2 with the overlap-add fused: the result of
     ``overlap_add(istft_frames(z3, weff, nperseg), win, nperseg, nperseg // 2,
-    first, count)`` (the same sums in the same order) without the frames in
-    memory (reference: the two kernels of istft_frames, _stft_metal.py:753-815).  z3: (rows, nseg,
+    first, count)`` to a few ulps (the window-power divide is folded into
+    per-sample weights) without the frames in memory (reference: the two kernels of istft_frames, _stft_metal.py:753-815).  z3: (rows, nseg,
     nperseg/2 + 1) complex64; returns (rows, count) float32."""
     if z3.dtype != torch.complex64 or z3.dim() != 3 or z3.shape[-1] != nperseg // 2 + 1:
         raise ValueError("istft_ola takes (rows, nseg, nperseg/2+1) complex64")

@@ -86,11 +86,10 @@ def _ola_cases():
 @pytest.mark.parametrize("nperseg,rows,nseg", _ola_cases())
 def test_fused_istft_equals_frames_then_overlap_add(rng, nperseg, rows, nseg):
     """The fused hop = N/2 kernel gives the two-kernel result, for the whole
-    output, the boundary-trimmed span and a random span: the same frame
-    values summed in the same order and divided by the same window power,
-    so the only difference is the inverse transform's own rounding (the
-    compiler may fuse a multiply-add differently in the two kernels):
-    bound 1e-6 x the peak |x|."""
+    output, the boundary-trimmed span and a random span.  It folds the
+    window-power divide into per-sample weights (weff / norm rounded once)
+    instead of dividing the sum, so the two differ by a few ulps: bound
+    1e-6 x the peak |x|."""
     dev = torch.device("cuda")
     m = nperseg // 2
     z = torch.from_numpy((rng.standard_normal((rows, nseg, m + 1))
