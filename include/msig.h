@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define MSIG_ABI_VERSION 5
+#define MSIG_ABI_VERSION 6
 
 #define MSIG_EINVAL (-22)
 #define MSIG_ENOTSUP (-95)
@@ -92,6 +92,13 @@ typedef struct msig_job {
   int64_t nvalid;  /* samples stored per row (0: n - lead).  Extended sample p
                       is stored sample p - lead, and reads as zero outside
                       [0, nvalid); ld_x / ld_y >= nvalid.                       */
+  const float* side_src; /* optional side copy (side_len 0: none): the call's   */
+  float* side_dst;       /* kernels also copy side_len floats from side_src to */
+  int64_t side_len;      /* side_dst (device buffers, not overlapping), so a
+                            caller returning cached constants alongside the
+                            result (scipy's frequency and time grids,
+                            spectral.py:301, 355-365) needs no copy launch of
+                            its own.  Single calls only (batch plans reject it). */
 } msig_job;
 
 /* ABI version the library was built with (MSIG_ABI_VERSION). */
@@ -264,11 +271,12 @@ int msig_istft_frames(const void* z, int64_t rows, int32_t nseg, int32_t nperseg
  * results differ from the two-call form by a few ulps.  z complex64 [rows][nseg][nperseg/2+1];
  * weff = window * scale / nperseg and win (the window, for the window-power
  * divide) float32[nperseg]; out float32 [rows][count] = samples first .. first+count-1,
- * first + count <= (nseg + 1) * nperseg/2.  nperseg a power of two in
+ * first + count <= (nseg + 1) * nperseg/2.  side_src / side_dst / side_len: an optional
+ * side copy as in msig_job (istft's time grid).  nperseg a power of two in
  * [MSIG_MIN_NFFT, MSIG_MAX_NFFT], else MSIG_ENOTSUP. */
 int msig_istft_ola(const void* z, int64_t rows, int32_t nseg, int32_t nperseg,
                    const float* weff, const float* win, int64_t first, int64_t count, float* out,
-                   void* stream);
+                   const float* side_src, float* side_dst, int64_t side_len, void* stream);
 
 /* Overlap-add with the window-power divide (reference: _stft_metal.py:793-815,
  * the gather kernel of istft_frames): frames float32 [rows][nseg][nperseg][channels]

@@ -128,6 +128,9 @@ struct FinArgs {
   int32_t bpr, N, nbins, nc, kind, fold, apply_factor;
   double scale;    // scale / nseg
   int64_t ws_off;  // batch plans: partials start at ws_base + ws_off
+  const float* side_src;  // side copy (block 0): side_len floats to side_dst
+  float* side_dst;
+  int64_t side_len;
 };
 
 constexpr int FIN_BINS = 64;   // output bins per block
@@ -214,6 +217,7 @@ __device__ __forceinline__ void finalize_write(const FinArgs& f, int64_t row, in
 }
 
 __global__ __launch_bounds__(FIN_BINS * FIN_LANES) void k_finalize(const FinArgs f) {
+  side_copy(f.side_src, f.side_dst, f.side_len);
   const int chunks = (f.nbins + FIN_BINS - 1) / FIN_BINS;
   finalize_chunk(f, blockIdx.x / chunks, (int)(blockIdx.x % chunks));
 }
@@ -603,6 +607,8 @@ static int check_job(const msig_job* j) {
   if (j->n < j->nperseg)
     return fail(MSIG_EINVAL, "n (%lld) must be >= nperseg (%d)", (long long)j->n, j->nperseg);
   if (j->lead < 0 || j->nvalid < 0) return fail(MSIG_EINVAL, "lead and nvalid must be >= 0");
+  if (j->side_len < 0 || (j->side_len > 0 && (!j->side_src || !j->side_dst)))
+    return fail(MSIG_EINVAL, "side copy: side_len must be >= 0 with both buffers given");
   const bool ext = j->lead != 0 || j->nvalid != 0;
   if (ext && j->kind != MSIG_FRAMES_COMPLEX)
     return fail(MSIG_ENOTSUP,
@@ -682,6 +688,9 @@ static KArgs base_args(const msig_job* j, const void* x, const void* y, const fl
   a.detrend = (j->flags & MSIG_F_DETREND_CONST) ? 1 : 0;
   a.lead = j->lead;
   a.nvalid = stored_len(j);
+  a.side_src = j->side_src;
+  a.side_dst = j->side_dst;
+  a.side_len = j->side_len;
   return a;
 }
 
@@ -789,6 +798,7 @@ int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const 
   });
   if (e != hipSuccess) return hip_fail(e, "fused mean kernel launch");
   FinArgs f;
+  std::memset(&f, 0, sizeof f);
   f.ws = a.ws;
   f.out0 = out0;
   f.out1 = out1;
@@ -803,6 +813,9 @@ int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const 
   f.apply_factor =
       (!cplx && (job->flags & MSIG_F_APPLY_FACTOR) && (job->flags & MSIG_F_ONESIDED)) ? 1 : 0;
   f.scale = job->scale / (double)job->nseg;
+  f.side_src = job->side_src;
+  f.side_dst = job->side_dst;
+  f.side_len = job->side_len;
   const int64_t chunks = (f.nbins + FIN_BINS - 1) / FIN_BINS;
   const dim3 grid((unsigned)(job->rows * chunks)), block(FIN_BINS * FIN_LANES);
   hipLaunchKernelGGL(k_finalize, grid, block, 0, s, f);
@@ -941,6 +954,7 @@ int msig_batch_create(const msig_batch_job* jobs, int32_t njobs, const msig_tuni
     if (!is_mean(j->kind)) return fail(MSIG_EINVAL, "job %d: kind %d is not a mean kind", i, j->kind);
     if (precision_of(j) != PREC_FP32)
       return fail(MSIG_ENOTSUP, "job %d: batch plans run the FP32 form only", i);
+    if (j->side_len) return fail(MSIG_EINVAL, "job %d: batch plans take no side copy", i);
     const bool pair = is_pair(j->kind);
     if (!bj.x || !bj.win || !bj.out0 || (pair && !bj.y))
       return fail(MSIG_EINVAL, "job %d: NULL input/output pointer", i);
@@ -1249,8 +1263,10 @@ int msig_istft_frames(const void* z, int64_t rows, int32_t nseg, int32_t nperseg
 
 int msig_istft_ola(const void* z, int64_t rows, int32_t nseg, int32_t nperseg,
                    const float* weff, const float* win, int64_t first, int64_t count, float* out,
-                   void* stream) {
+                   const float* side_src, float* side_dst, int64_t side_len, void* stream) {
   if (!z || !weff || !win || !out) return fail(MSIG_EINVAL, "NULL pointer");
+  if (side_len < 0 || (side_len > 0 && (!side_src || !side_dst)))
+    return fail(MSIG_EINVAL, "side copy: side_len must be >= 0 with both buffers given");
   if (rows < 1 || nseg < 1) return fail(MSIG_EINVAL, "rows and nseg must be >= 1");
   if (!pow2(nperseg) || nperseg < MSIG_MIN_NFFT || nperseg > MSIG_MAX_NFFT)
     return fail(MSIG_ENOTSUP, "nperseg %d outside the fused power-of-two range [%d, %d]",
@@ -1282,6 +1298,9 @@ int msig_istft_ola(const void* z, int64_t rows, int32_t nseg, int32_t nperseg,
   a.nbins = N / 2 + 1;
   a.ola_first = first;
   a.ola_count = count;
+  a.side_src = side_src;
+  a.side_dst = side_dst;
+  a.side_len = side_len;
   hipError_t e = dispatch_istft_ola(N, a, rows, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "fused istft kernel launch");
   return 0;

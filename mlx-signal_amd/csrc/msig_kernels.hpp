@@ -227,6 +227,15 @@ __device__ __forceinline__ void prep_unit(cf* z, const float* w, int detrend, in
   }
 }
 
+// A caller's side copy (cached constants returned beside the result), done
+// by a kernel the call launches anyway: grid-strided over all its blocks
+// (istft's time grid is as long as the signal).
+__device__ __forceinline__ void side_copy(const float* src, float* dst, int64_t len) {
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += step)
+    as_global_mut(dst)[i] = as_global(src)[i];
+}
+
 // Grouped mean over segments: per-bin sums of |Zx|^2 (PSD), conj(Zx) Zy
 // (CSD) or both spectra and the cross term (COH), written per block.
 template <int N, int E, int KIND, bool CPLX, bool HALF, int S = 0>
@@ -675,6 +684,7 @@ __global__ __launch_bounds__(BT, (frames_waves<N, OK>())) void k_frames(const KA
   __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
   __shared__ float red[2 * (BT / 64)];
   __shared__ cf twl[TH::LDS_ELEMS];
+  side_copy(a.side_src, a.side_dst, a.side_len);
   if constexpr (TH::LDS) {
     fill_twiddle_lds<N, E>(twl, a.twt);
     __syncthreads();
@@ -923,6 +933,7 @@ __global__ __launch_bounds__(BT, (N >= 4096 ? 1 : 2)) void k_istft_ola(const KAr
   using TH = TwiddleHome<N, E, 0, frames_twl<N>()>;
   __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
   __shared__ cf twl[TH::LDS_ELEMS];
+  side_copy(a.side_src, a.side_dst, a.side_len);
   if constexpr (TH::LDS) {
     fill_twiddle_lds<N, E>(twl, a.twt);
     __syncthreads();

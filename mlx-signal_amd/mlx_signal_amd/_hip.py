@@ -17,7 +17,7 @@ import os
 
 # MSIG_LIBRARY selects another build of the library (e.g. a tuning variant)
 LIB_PATH = Path(os.environ.get("MSIG_LIBRARY") or Path(__file__).with_name("libmsig.so"))
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 MSIG_EINVAL = -22
 MSIG_ENOTSUP = -95
@@ -55,6 +55,9 @@ class MsigJob(ctypes.Structure):
         ("scale", ctypes.c_double),
         ("lead", ctypes.c_int64),
         ("nvalid", ctypes.c_int64),
+        ("side_src", ctypes.c_void_p),
+        ("side_dst", ctypes.c_void_p),
+        ("side_len", ctypes.c_int64),
     ]
 
 
@@ -129,7 +132,8 @@ SIGNATURES = {
                                            ctypes.POINTER(_i32), _vp]),
     "msig_istft_frames": (ctypes.c_int, [_vp, _i64, _i32, _i32, _vp, _vp, _vp]),
     "msig_overlap_add": (ctypes.c_int, [_vp, _i64, _i32, _i32, _i32, _i32, _vp, _i64, _i64, _vp, _vp]),
-    "msig_istft_ola": (ctypes.c_int, [_vp, _i64, _i32, _i32, _vp, _vp, _i64, _i64, _vp, _vp]),
+    "msig_istft_ola": (ctypes.c_int, [_vp, _i64, _i32, _i32, _vp, _vp, _i64, _i64, _vp, _vp, _vp,
+                                      _i64, _vp]),
     "msig_median_frames": (ctypes.c_int, [_vp, _i64, _i32, _i32, _i32, ctypes.c_double, _i32, _i32, _vp, _vp]),
     "msig_kernel_timing_arm": (ctypes.c_int, [_i32]),
     "msig_kernel_timing_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i32)]),

@@ -120,12 +120,30 @@ def mfma_eligible(kind: str, x2, nperseg: int, nfft: int) -> bool:
             and nfft == _hip.MFMA_NFFT)
 
 
+def _side(side, dev):
+    """(src, dst) float32 device tensors of one length, or None: the side copy
+    a call's kernels make (cached constants returned beside a result)."""
+    if side is None:
+        return 0, 0, 0
+    src, dst = side
+    for t in (src, dst):
+        if t.dtype != torch.float32 or t.device != dev or not t.is_contiguous():
+            raise ValueError(f"side copy buffers must be contiguous float32 tensors on {dev}")
+    if src.numel() != dst.numel():
+        raise ValueError("side copy buffers differ in length")
+    return src.data_ptr(), dst.data_ptr(), src.numel()
+
+
 def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, onesided=True,
-                  apply_factor=False, scale=1.0, precision="fp32"):
+                  apply_factor=False, scale=1.0, precision="fp32", side=None):
     """Mean over segments of |X|^2 ("psd"), conj(X) Y ("csd"), the
     (Pxx, Pyy, Pxy) trio ("coh_trio") or |Pxy|^2/(Pxx Pyy) ("coherence"),
     times ``scale`` (and the onesided factor when ``apply_factor``).
     x2 / y2: (rows, n).  Returns a tuple of (rows, bins) tensors.
+
+    ``side``: optional (src, dst) float32 device tensors; the call's kernels
+    also copy src into dst (the caller's cached frequency grid), so
+    returning it takes no copy launch of its own.
 
     ``precision`` "fp16_split" / "bf16_split" runs the matrix-core form where
     :func:`mfma_eligible` holds and the FP32 kernels elsewhere."""
@@ -159,6 +177,7 @@ def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, oneside
         outs = (torch.empty(rows, bins, **f32), torch.empty(rows, bins, **f32),
                 torch.empty(rows, bins, **c64))
     o = list(outs) + [None] * (3 - len(outs))
+    job.side_src, job.side_dst, job.side_len = _side(side, dev)
     rc = lib.msig_spectral_mean(ctypes.byref(job), _hip.ptr(x2), _hip.ptr(y2), _hip.ptr(win),
                                 _hip.ptr(o[0]), _hip.ptr(o[1]), _hip.ptr(o[2]), _hip.ptr(ws),
                                 ws.numel(), _hip.stream_of(x2))
@@ -168,12 +187,13 @@ def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, oneside
 
 
 def spectral_frames(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, onesided=True,
-                    apply_factor=False, scale=1.0, lead=0, n_ext=None):
+                    apply_factor=False, scale=1.0, lead=0, n_ext=None, side=None):
     """Per-segment X ("complex", times sqrt(scale)), |X|^2 ("power"),
     conj(X) Y ("csd") or the trio ("coh"), times ``scale``.  Returns
     (rows, nseg, bins) tensors.  lead / n_ext: segments are cut from each row
     extended by `lead` zeros in front and zeros after it to n_ext samples
-    (the kernel reads the zeros; nothing is copied)."""
+    (the kernel reads the zeros; nothing is copied).  ``side``: an optional
+    (src, dst) side copy, as for :func:`spectral_mean`."""
     x2 = _rows(x2)
     y2 = None if y2 is None else _rows(y2)
     _check_inputs(x2, y2, win, nperseg)
@@ -188,6 +208,7 @@ def spectral_frames(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, onesi
     rows = x2.shape[0]
     nseg = job.nseg
     dev = x2.device
+    job.side_src, job.side_dst, job.side_len = _side(side, dev)
     f32 = dict(dtype=torch.float32, device=dev)
     c64 = dict(dtype=torch.complex64, device=dev)
     if code == _hip.FRAMES_POWER:
@@ -388,21 +409,27 @@ def overlap_add(frames, win, nperseg: int, hop: int, first: int, count: int):
     return torch.view_as_complex(out) if cplx else out
 
 
-def istft_ola(z3, weff, win, nperseg: int, first: int, count: int):
+def istft_ola(z3, weff, win, nperseg: int, first: int, count: int, side=None):
     """istft for hop = nperseg/2 with the overlap-add fused: the result of
     ``overlap_add(istft_frames(z3, weff, nperseg), win, nperseg, nperseg // 2,
     first, count)`` to a few ulps (the window-power divide is folded into
-    per-sample weights) without the frames in memory (reference: the two kernels of istft_frames, _stft_metal.py:753-815).  z3: (rows, nseg,
-    nperseg/2 + 1) complex64; returns (rows, count) float32."""
+    per-sample weights) without the frames in memory (reference: the two
+    kernels of istft_frames, _stft_metal.py:753-815).  z3: (rows, nseg,
+    nperseg/2 + 1) complex64; returns (rows, count) float32.  ``side``: an
+    optional (src, dst) side copy, as for :func:`spectral_mean`."""
     if z3.dtype != torch.complex64 or z3.dim() != 3 or z3.shape[-1] != nperseg // 2 + 1:
         raise ValueError("istft_ola takes (rows, nseg, nperseg/2+1) complex64")
     z3 = z3.contiguous()
     rows, nseg, _ = z3.shape
     out = torch.empty(rows, count, dtype=torch.float32, device=z3.device)
     if out.numel() == 0:
+        if side is not None:
+            side[1].copy_(side[0])
         return out
+    src, dst, n = _side(side, z3.device)
     rc = _hip.load().msig_istft_ola(_hip.ptr(z3), rows, nseg, nperseg, _hip.ptr(weff),
                                    _hip.ptr(win), first, count, _hip.ptr(out),
+                                   ctypes.c_void_p(src), ctypes.c_void_p(dst), n,
                                    _hip.stream_of(z3))
     _hip.check(rc, "istft_ola")
     launches["istft_ola"] += 1

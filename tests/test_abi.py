@@ -125,7 +125,8 @@ def test_job_struct_matches_header():
     """msig_job's fields, in order, are the ctypes structure's."""
     text = (ROOT / "include" / "msig.h").read_text()
     body = re.search(r"typedef struct msig_job \{(.*?)\} msig_job;", text, re.S).group(1)
-    fields = re.findall(r"^\s*(?:int32_t|int64_t|double)\s+(\w+);", body, re.M)
+    fields = re.findall(r"^\s*(?:int32_t|int64_t|double|float\*|const float\*)\s+(\w+);",
+                        body, re.M)
     assert fields == [f for f, _ in _hip.MsigJob._fields_]
 
 
@@ -158,12 +159,34 @@ def test_fused_istft_rejects_bad_arguments_without_gpu():
         pytest.skip("libmsig.so not built")
     lib = _hip.load()
     p = ctypes.c_void_p(16)
-    assert lib.msig_istft_ola(None, 1, 4, 256, p, p, 0, 10, p, None) == _hip.MSIG_EINVAL
-    assert lib.msig_istft_ola(p, 0, 4, 256, p, p, 0, 10, p, None) == _hip.MSIG_EINVAL
-    assert lib.msig_istft_ola(p, 1, 4, 100, p, p, 0, 10, p, None) == _hip.MSIG_ENOTSUP
-    assert lib.msig_istft_ola(p, 1, 4, 8192, p, p, 0, 10, p, None) == _hip.MSIG_ENOTSUP
+    def call(z, rows, nperseg, first, count):
+        return lib.msig_istft_ola(z, rows, 4, nperseg, p, p, first, count, p, None, None, 0, None)
+
+    assert call(None, 1, 256, 0, 10) == _hip.MSIG_EINVAL
+    assert call(p, 0, 256, 0, 10) == _hip.MSIG_EINVAL
+    assert call(p, 1, 100, 0, 10) == _hip.MSIG_ENOTSUP
+    assert call(p, 1, 8192, 0, 10) == _hip.MSIG_ENOTSUP
     # the output spans (nseg + 1) * nperseg / 2 = 640 samples
-    assert lib.msig_istft_ola(p, 1, 4, 256, p, p, 1, 640, p, None) == _hip.MSIG_EINVAL
+    assert call(p, 1, 256, 1, 640) == _hip.MSIG_EINVAL
     assert b"outside" in lib.msig_last_error()
-    assert lib.msig_istft_ola(p, 1, 4, 256, p, p, -1, 10, p, None) == _hip.MSIG_EINVAL
-    assert lib.msig_istft_ola(p, 1, 4, 256, p, p, 640, 0, p, None) == 0  # nothing to do
+    assert call(p, 1, 256, -1, 10) == _hip.MSIG_EINVAL
+    assert call(p, 1, 256, 640, 0) == 0  # nothing to do
+
+
+def test_side_copy_arguments_checked_without_gpu():
+    """A side copy needs both buffers and a non-negative length; batch plans
+    take none (checked before any device call)."""
+    if not _hip.LIB_PATH.exists():
+        pytest.skip("libmsig.so not built")
+    lib = _hip.load()
+    job = _hip.MsigJob(_hip.PSD_MEAN, _hip.F_ONESIDED, 1, 1024, 1024, 1024, 256, 128, 256,
+                       7, 1.0, 0, 0, 16, None, 8)
+    sz = ctypes.c_size_t(0)
+    assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == _hip.MSIG_EINVAL
+    assert b"side copy" in lib.msig_last_error()
+    job.side_dst = 32
+    assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == 0
+    job.side_len = -1
+    assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == _hip.MSIG_EINVAL
+    p = ctypes.c_void_p(16)
+    assert lib.msig_istft_ola(p, 1, 4, 256, p, p, 0, 10, p, p, None, 4, None) == _hip.MSIG_EINVAL

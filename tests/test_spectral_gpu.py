@@ -467,3 +467,46 @@ def test_stft_reads_boundary_zeros_in_place(rng):
     assert _kernels.launches["frames_complex"] == before + 1
     ref = oracle.stft(x, nperseg=512)[2]
     assert_peak_close(z, ref, TOL)
+
+
+@pytest.mark.parametrize("fs", [1.0, 48000.0, 250.0, 44100.0, 1e-3, 7.3])
+@pytest.mark.parametrize("nperseg", [16, 256, 1000, 1024, 4096])
+def test_grids_are_fresh_and_exact(rng, fs, nperseg):
+    """The frequency and time grids returned beside each result (copied from
+    the cached constants by the call's own kernels on the fused paths) are
+    scipy's, bit for bit, and owned by the caller: mutating them leaves the
+    next call unaffected."""
+    n = 3 * nperseg + 5
+    x = make_sig(rng, (2, n))
+    y = make_sig(rng, (2, n))
+    fref = np.fft.rfftfreq(nperseg, 1 / fs).astype(np.float32)
+    calls = [(lambda: msig.welch(x, fs=fs, nperseg=nperseg), oracle.welch(x, fs=fs, nperseg=nperseg)),
+             (lambda: msig.csd(x, y, fs=fs, nperseg=nperseg), oracle.csd(x, y, fs=fs, nperseg=nperseg)),
+             (lambda: msig.coherence(x, y, fs=fs, nperseg=nperseg),
+              oracle.coherence(x, y, fs=fs, nperseg=nperseg)),
+             (lambda: msig.welch(x, fs=fs, nperseg=nperseg, average="median"),
+              oracle.welch(x, fs=fs, nperseg=nperseg, average="median")),
+             (lambda: msig.spectrogram(x, fs=fs, nperseg=nperseg),
+              oracle.spectrogram(x, fs=fs, nperseg=nperseg)),
+             (lambda: msig.stft(x, fs=fs, nperseg=nperseg), oracle.stft(x, fs=fs, nperseg=nperseg))]
+    for fn, ref in calls:
+        got = fn()
+        grids = got[:-1]
+        for g, r in zip(grids, ref[:-1]):
+            np.testing.assert_array_equal(as_np(g), np.asarray(r).astype(np.float32))
+            g.fill_(-1.0)  # the caller owns it
+        for g, r in zip(fn()[:-1], ref[:-1]):
+            np.testing.assert_array_equal(as_np(g), np.asarray(r).astype(np.float32))
+    np.testing.assert_array_equal(as_np(calls[0][0]()[0]), fref)
+    xc = (x + 1j * y).astype(np.complex64)  # two-sided
+    f, _ = msig.welch(xc, fs=fs, nperseg=nperseg)
+    np.testing.assert_array_equal(as_np(f), np.fft.fftfreq(nperseg, 1 / fs).astype(np.float32))
+    f, _ = msig.welch(x, fs=fs, nperseg=nperseg, return_onesided=False)
+    np.testing.assert_array_equal(as_np(f), np.fft.fftfreq(nperseg, 1 / fs).astype(np.float32))
+    _, _, z = msig.stft(x, fs=fs, nperseg=nperseg)
+    t, _ = msig.istft(z, fs=fs, nperseg=nperseg)
+    t_ref, _ = oracle.istft(z.cpu().numpy(), fs=fs, nperseg=nperseg)
+    np.testing.assert_array_equal(as_np(t), t_ref.astype(np.float32))
+    t.fill_(-1.0)
+    np.testing.assert_array_equal(as_np(msig.istft(z, fs=fs, nperseg=nperseg)[0]),
+                                  t_ref.astype(np.float32))
