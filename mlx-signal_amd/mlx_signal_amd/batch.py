@@ -69,8 +69,9 @@ _KINDS = {"welch": _hip.PSD_MEAN, "csd": _hip.CSD_MEAN, "coherence": _hip.COHERE
 
 class SpectralBatch:
     """A planned ragged batch.  ``run()`` launches it on the current stream;
-    ``outputs[i]`` is request i's result tensor ((..., bins)), ``freqs[i]``
-    its frequency grid."""
+    ``outputs[i]`` is request i's result tensor ((..., bins)), rewritten by
+    every run; ``freqs[i]`` its frequency grid, the device-cached constant
+    shared with other requests and calls (read-only: clone it to modify)."""
 
     def __init__(self, requests, tuning=None):
         self.device = compute_device()
