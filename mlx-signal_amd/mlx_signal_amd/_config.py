@@ -27,6 +27,8 @@ import contextlib
 import dataclasses
 import warnings
 
+from ._device import gpu_available
+
 __all__ = [
     "Config",
     "DowncastWarning",
@@ -66,6 +68,11 @@ _config = Config()
 def get_config() -> Config:
     """Return a copy of the current global configuration."""
     return dataclasses.replace(_config)
+
+
+def precision() -> str:
+    """The configured arithmetic form of the fused means (no copy: hot path)."""
+    return _config.precision
 
 
 def set_config(**kwargs) -> None:
@@ -108,8 +115,6 @@ def use_gpu(work_size: int) -> bool:
         return True
     if _config.dispatch == "scipy":
         return False
-    from ._device import gpu_available
-
     return work_size >= _config.gpu_min_size and gpu_available()
 
 

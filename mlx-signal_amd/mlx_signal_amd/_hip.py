@@ -181,8 +181,15 @@ def check(rc: int, what: str) -> None:
     raise RuntimeError(text)
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream_of(t: torch.Tensor):
-    """The current HIP stream of t's device, as the raw handle the ABI takes."""
+    """The current HIP stream of t's device, as the raw handle the ABI takes
+    (torch's raw accessor when present: the Stream object costs a few us)."""
+    if _raw_stream is not None:
+        idx = t.device.index
+        return ctypes.c_void_p(_raw_stream(torch.cuda.current_device() if idx is None else idx))
     return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
 

@@ -120,6 +120,11 @@ def mfma_eligible(kind: str, x2, nperseg: int, nfft: int) -> bool:
             and nfft == _hip.MFMA_NFFT)
 
 
+# Workspace bytes per (kind, flags, rows, nseg, nfft): the block partials'
+# size is a function of the launch geometry, which depends on nothing else.
+_WS_BYTES: dict = {}
+
+
 def _side(side, dev):
     """(src, dst) float32 device tensors of one length, or None: the side copy
     a call's kernels make (cached constants returned beside a result)."""
@@ -164,9 +169,15 @@ def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, oneside
     bins = nfft // 2 + 1 if (onesided and not cplx) else nfft
     rows = x2.shape[0]
     dev = x2.device
-    ws_bytes = ctypes.c_size_t(0)
-    _hip.check(lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(ws_bytes)), kind)
-    ws = _workspace(dev, ws_bytes.value)
+    key = (job.kind, job.flags, job.rows, job.nseg, job.nfft)  # what the partials depend on
+    nbytes = _WS_BYTES.get(key)
+    if nbytes is None:
+        ws_bytes = ctypes.c_size_t(0)
+        _hip.check(lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(ws_bytes)), kind)
+        nbytes = _WS_BYTES[key] = ws_bytes.value
+        if len(_WS_BYTES) > 4096:
+            _WS_BYTES.clear()
+    ws = _workspace(dev, nbytes)
     f32 = dict(dtype=torch.float32, device=dev)
     c64 = dict(dtype=torch.complex64, device=dev)
     if code == _hip.PSD_MEAN or code == _hip.COHERENCE:
