@@ -235,3 +235,21 @@ def test_istft_scipy_route_on_cpu():
     assert isinstance(xr, torch.Tensor) and isinstance(t, torch.Tensor)
     _, x_ref = oracle.istft(z, nperseg=128)
     np.testing.assert_allclose(xr.numpy(), x_ref, atol=1e-5 * np.abs(x_ref).max())
+
+
+def test_window_sums_cached_only_for_read_only_windows():
+    """The planner keeps (sum w, sum w^2) per named-window array (read-only,
+    from the window cache) and recomputes them for a caller's own array,
+    which may change between calls."""
+    from mlx_signal_amd.windows import _window_np
+
+    w = _window_np("hann", 256)
+    assert not w.flags.writeable
+    s1 = spectral._win_sums(w)
+    assert s1 == (float(w.sum()), float((w * w).sum()))
+    assert spectral._win_sums(w) is s1  # cached for the same read-only array
+    mine = np.hanning(256)
+    a = spectral._win_sums(mine)
+    mine *= 2.0
+    b = spectral._win_sums(mine)
+    assert b[0] == pytest.approx(2 * a[0]) and b[1] == pytest.approx(4 * a[1])
