@@ -478,8 +478,8 @@ def main():
                          "FP16/BF16 split) and print its JSON; for profiling that kernel")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by channel with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (the path "
+              "would shard by channel row with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import torch

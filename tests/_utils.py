@@ -23,7 +23,7 @@ def make_sig(rng, shape, complex_=False):
 
 
 def peak_rel_err(actual, desired) -> float:
-    """max |a - d| / max |d| (the north-star metric: error relative to the peak bin)."""
+    """max |a - d| / max |d| (the parity metric: error relative to the peak bin)."""
     a = as_np(actual).astype(np.complex128)
     d = np.asarray(desired).astype(np.complex128)
     assert a.shape == d.shape, f"shape mismatch: {a.shape} vs {d.shape}"

@@ -17,7 +17,7 @@ from mlx_signal_amd import _hip, _kernels
 pytestmark = pytest.mark.gpu
 
 # per-bin error relative to the FP64 peak bin.  FP16 hi + lo carries ~22
-# significant bits per operand: the north-star FP32 bound holds.  BF16 hi + lo
+# significant bits per operand: the FP32 path's 1e-5 bound holds.  BF16 hi + lo
 # carries ~16: the stated bound is 2e-5.
 TOLS = {"fp16_split": 1e-5, "bf16_split": 2e-5}
 
