@@ -201,7 +201,7 @@ def cpu_all_cores():
 
 
 def bench_precision(torch, msig, dev, steps=20):
-    """Configuration 3: welch over 256 channels x 2^16 (N(0,1) plus one tone
+    """Precision forms: welch over 256 channels x 2^16 (N(0,1) plus one tone
     per channel), nperseg 256, noverlap 128, hann: the FP32 kernels vs the
     matrix-core FP16 / BF16 hi+lo forms, with max and RMS per-bin error
     relative to each channel's FP64 peak (8 channels against the oracle)."""
@@ -243,7 +243,7 @@ def bench_precision(torch, msig, dev, steps=20):
 
 
 def bench_small(torch, msig, dev, steps=50):
-    """Configuration 1: welch over 1 x 2^16 float32 N(0,1), nperseg 256,
+    """Small welch: 1 x 2^16 float32 N(0,1), nperseg 256,
     noverlap 128 (launch-latency bound), beside scipy on one host core."""
     import oracle
 
@@ -292,7 +292,7 @@ def ragged_requests(torch, dev, n_inst=512, seed=7):
 
 
 def bench_ragged(torch, msig, dev, reps=3):
-    """Configuration 4: one plain call per request vs one ragged batch
+    """Ragged batch of 512 instances: one plain call per request vs one ragged batch
     (autotuned, and replayed from a HIP graph)."""
     import oracle
     from mlx_signal_amd.batch import SpectralBatch
@@ -423,7 +423,7 @@ def _stats(done):
 
 
 def bench_serving(torch, msig, dev):
-    """Configuration 5: Poisson 2000 req/s; continuous batching vs one plain
+    """Serving: Poisson 2000 req/s; continuous batching vs one plain
     call per request, plus saturation throughput (all requests at t = 0)."""
     from mlx_signal_amd.serving import SpectralServer, latency_stats
     from mlx_signal_amd.tuning import cached_tuning
