@@ -507,9 +507,12 @@ constexpr int mean_waves() {
 }
 // The extension stream at N = 1024 spilled at three waves per SIMD; two run
 // the stft 16 x 2^20 kernel in 48 us against 53 (at 512, three stay faster).
-template <int N, int OK, bool EXT = false>
+// The coherence trio on the general stream spilled 105-186 VGPRs at two
+// waves from N = 512; one runs it 11% faster (csd frames stay at two).
+template <int N, int OK, bool EXT = false, bool HALF = true>
 constexpr int frames_waves() {
   if (EXT && N == 1024) return 2;
+  if (!HALF && N >= 512 && OK == F_COH) return 1;
   return N >= 2048 ? ((OK == F_COMPLEX || OK == F_POWER) ? 2 : 1)
                    : ((OK == F_COMPLEX || OK == F_POWER) ? 3 : 2);
 }
@@ -681,7 +684,7 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
 
 // EXT: rows read through the virtual zero extension (stft's boundary zeros)
 template <int N, int E, int OK, bool CPLX, bool HALF, bool EXT = false>
-__global__ __launch_bounds__(BT, (frames_waves<N, OK, EXT>())) void k_frames(const KArgs a) {
+__global__ __launch_bounds__(BT, (frames_waves<N, OK, EXT, HALF>())) void k_frames(const KArgs a) {
   using PL = Plan<N, E>;
   using TH = TwiddleHome<N, E, 0, frames_twl<N>()>;
   __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
