@@ -496,13 +496,17 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
 // waves with 14-34 spilled VGPRs measured 281 -> 183 us on 64 x 2^20 at
 // N = 1024, 364 -> 231 us at 2048 and 362 -> 251 us at 4096 (twiddles in
 // LDS there), where one wave left the latencies exposed).
-template <int N, int KIND, bool HALF>
+// CPLX: complex input.  The real-input csd on the general stream spilled
+// 22 VGPRs at two waves from N = 512 (1024: 389 -> 339 us at one wave over
+// 64 x 2^20 with noverlap 3N/4; 512: 335 -> 329).
+template <int N, int KIND, bool HALF, bool CPLX = false>
 constexpr int mean_waves() {
   if constexpr (N >= 2048) {
     if constexpr (KIND == K_PSD) return (N == 2048 && HALF) ? 3 : 2;
     if constexpr (KIND == K_CSD) return (N < 4096 || HALF) ? 2 : 1;
     return HALF ? 2 : 1;
   }
+  if constexpr (KIND == K_CSD && !HALF && !CPLX && N >= 512) return 1;
   return KIND == K_PSD ? 3 : (KIND == K_CSD ? 2 : (HALF ? 2 : 1));
 }
 // The extension stream at N = 1024 spilled at three waves per SIMD; two run
@@ -519,7 +523,7 @@ constexpr int frames_waves() {
 
 // HALF selects the hop == N/2 register-reuse stream (real input, nperseg == N).
 template <int N, int E, int KIND, bool CPLX, bool HALF>
-__global__ __launch_bounds__(BT, (mean_waves<N, KIND, HALF>())) void k_mean(const KArgs a) {
+__global__ __launch_bounds__(BT, (mean_waves<N, KIND, HALF, CPLX>())) void k_mean(const KArgs a) {
   using PL = Plan<N, E>;
   using TH = TwiddleHome<N, E, 0, mean_twl<N, KIND, HALF>()>;
   __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
