@@ -2,7 +2,8 @@
 against the FP64 oracle (itself pinned to scipy.signal): nperseg (powers of
 two across the fused range and a few others that take the composed path),
 noverlap, nfft padding, detrend, scaling, window, sides, average, axis,
-complex input and DC-offset signals.  Frequency and time grids are exact.
+complex input and DC-offset signals; and istft over sizes, windows, hops and
+boundary trimming.  Frequency and time grids are exact.
 
 Bound: 1e-5 x the peak |ref| bin, or, where a case is ill-conditioned in
 FP32 (a Kaiser window's tiny edge weights, a coherence ratio over few
@@ -110,3 +111,22 @@ def test_random_coherence_spectrogram_stft(seed):
     np.testing.assert_array_equal(as_np(f), f_ref.astype(np.float32))
     np.testing.assert_array_equal(as_np(t), t_ref.astype(np.float32))
     check(z, z_ref, lambda: sps.stft(x, **opts)[2])
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_istft(seed):
+    rng = np.random.default_rng(5000 + seed)
+    nps = int(rng.choice([16, 64, 256, 1024, 4096, 100, 48]))
+    window = ["hann", "hamming", "blackman", ("tukey", 0.5)][int(rng.integers(4))]
+    noverlap = [nps // 2, (3 * nps) // 4, nps // 2 + 1][int(rng.integers(3))]
+    boundary = bool(rng.random() < 0.7)
+    kw = dict(fs=float(rng.choice([1.0, 48000.0])), window=window, nperseg=nps, noverlap=noverlap)
+    x = rng.standard_normal((int(rng.integers(1, 4)), nps * int(rng.integers(2, 7)) + 7))
+    _, _, z = sps.stft(x, boundary="zeros" if boundary else None, padded=boundary, **kw)
+    z = z.astype(np.complex64)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")  # windows without boundary padding can fail NOLA
+        t, xr = msig.istft(z, boundary=boundary, **kw)
+        t_ref, x_ref = oracle.istft(z, boundary=boundary, **kw)
+    np.testing.assert_array_equal(as_np(t), t_ref.astype(np.float32))
+    check(xr, x_ref, lambda: sps.istft(z, boundary=boundary, **kw)[1])
