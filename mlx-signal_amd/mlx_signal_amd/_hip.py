@@ -184,6 +184,13 @@ def check(rc: int, what: str) -> None:
 _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
 
 
+def stream_of_device(dev: torch.device) -> int:
+    """The current HIP stream handle of `dev` as an integer."""
+    if _raw_stream is not None:
+        return _raw_stream(torch.cuda.current_device() if dev.index is None else dev.index)
+    return torch.cuda.current_stream(dev).cuda_stream
+
+
 def stream_of(t: torch.Tensor):
     """The current HIP stream of t's device, as the raw handle the ABI takes
     (torch's raw accessor when present: the Stream object costs a few us)."""

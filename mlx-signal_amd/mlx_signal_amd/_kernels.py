@@ -67,7 +67,9 @@ _WORKSPACES: dict = {}
 
 
 def _workspace(dev: torch.device, nbytes: int) -> torch.Tensor:
-    key = (dev.index, torch.cuda.current_stream(dev).cuda_stream)
+    """Block-partials buffer, cached per (device, current stream): calls on one
+    stream run in order, so they can share it; other streams get their own."""
+    key = (dev.index, _hip.stream_of_device(dev))
     buf = _WORKSPACES.get(key)
     if buf is None or buf.numel() < nbytes:
         buf = torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, device=dev)
