@@ -167,7 +167,7 @@ class SpectralBatch:
     def run(self):
         """Launch the batch on the current stream of its device."""
         lib = _hip.load()
-        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        stream = ctypes.c_void_p(_hip.stream_of_device(self.device))
         rc = lib.msig_batch_run(self._handle, _hip.ptr(self.descriptors), _hip.ptr(self.workspace),
                                 self.workspace.numel(), stream)
         _hip.check(rc, "batch run")
