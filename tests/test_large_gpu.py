@@ -43,3 +43,28 @@ def test_spectrogram_output_past_2_31_elements():
         assert torch.equal(s[r], s_row[0]), r
     del x, s
     torch.cuda.empty_cache()
+
+
+def test_csd_and_istft_past_2_31_elements():
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    n = 1 << 30
+    x = torch.randn(3, n, device=dev, generator=g)
+    y = 0.5 * x + torch.randn(3, n, device=dev, generator=g)
+    _, c = msig.csd(x, y, nperseg=512)
+    _, c_row = msig.csd(x[2:3].clone(), y[2:3].clone(), nperseg=512)
+    assert torch.equal(c[2], c_row[0])
+    del x, y, c, c_row
+    torch.cuda.empty_cache()
+    rows, n = 16, 1 << 27
+    x = torch.randn(rows, n, device=dev, generator=g)
+    _, _, z = msig.stft(x, nperseg=256)  # (16, 129, 2^20 + 1) complex: 2.2e9 values
+    assert z.numel() > (1 << 31)
+    _, xr = msig.istft(z, nperseg=256)
+    _, xr_row = msig.istft(z[rows - 1:rows].contiguous(), nperseg=256)
+    assert torch.equal(xr[rows - 1], xr_row[0])
+    err = float((xr[rows - 1, :n] - x[rows - 1]).abs().max() / x[rows - 1].abs().max())
+    assert err < 1e-5, err
+    del x, z, xr, xr_row
+    torch.cuda.empty_cache()
