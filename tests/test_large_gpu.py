@@ -68,3 +68,17 @@ def test_csd_and_istft_past_2_31_elements():
     assert err < 1e-5, err
     del x, z, xr, xr_row
     torch.cuda.empty_cache()
+
+
+def test_median_frames_past_2_31_elements():
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(6)
+    rows, n = 1 << 14, 1 << 18  # per-frame power: 2^14 x 2047 x 129 = 4.3e9 values
+    x = torch.randn(rows, n, device=dev, generator=g)
+    _, p = msig.welch(x, nperseg=256, average="median")
+    for r in (0, rows - 1):
+        _, p_row = msig.welch(x[r:r + 1].clone(), nperseg=256, average="median")
+        assert torch.equal(p[r], p_row[0]), r
+    del x, p
+    torch.cuda.empty_cache()
