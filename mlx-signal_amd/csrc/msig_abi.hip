@@ -128,7 +128,7 @@ struct FinArgs {
   int32_t bpr, N, nbins, nc, kind, fold, apply_factor;
   double scale;    // scale / nseg
   int64_t ws_off;  // batch plans: partials start at ws_base + ws_off
-  const float* side_src;  // side copy (block 0): side_len floats to side_dst
+  const float* side_src;  // side copy (grid-strided): side_len floats to side_dst
   float* side_dst;
   int64_t side_len;
 };

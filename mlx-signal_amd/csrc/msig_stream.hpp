@@ -29,7 +29,7 @@ struct KArgs {
   int64_t lead;      // zero samples virtually in front of each row (frames kinds)
   int64_t nvalid;    // samples stored per row; the extended row reads zero past them
   int64_t ola_first, ola_count;  // fused istft: output samples [first, first + count)
-  const float* side_src;  // side copy done by block 0: side_len floats to side_dst
+  const float* side_src;  // side copy, grid-strided over all blocks: side_len floats to side_dst
   float* side_dst;
   int64_t side_len;
 };
