@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define MSIG_ABI_VERSION 6
+#define MSIG_ABI_VERSION 7
 
 #define MSIG_EINVAL (-22)
 #define MSIG_ENOTSUP (-95)
@@ -123,6 +123,14 @@ int msig_eligible(int32_t nperseg, int32_t nfft, int32_t onesided, int32_t detre
 
 /* Workspace bytes the mean kinds need (0 for per-frame kinds). */
 int msig_workspace_bytes(const msig_job* job, size_t* bytes);
+
+/* Launch plan of a mean job on the current device: plan4 = {units per row
+ * (segment pairs for real input), FFT groups per block, units per group,
+ * blocks per row}.  Diagnostic, no counterpart in the reference (whose
+ * group parameters are internal, _stft_metal.py:429-441): tests use it to
+ * pick shapes whose groups walk several units and whose last block is
+ * uneven. */
+int msig_mean_plan(const msig_job* job, int32_t* plan4);
 
 /* Generic entry points (all kinds).  y may be NULL for single-signal kinds;
  * win is float32[nperseg] on the device.  The mean kinds write

@@ -657,10 +657,63 @@ static int check_precision(const msig_job* j) {
   return 0;
 }
 
+static int mean_kk(int kind) {
+  return kind == MSIG_PSD_MEAN ? K_PSD : (kind == MSIG_CSD_MEAN ? K_CSD : K_COH);
+}
+
+static bool job_half(const msig_job* j) {
+  return !(j->flags & MSIG_F_COMPLEX_INPUT) && j->nperseg == j->nfft && 2 * j->nstep == j->nfft;
+}
+
+// Blocks the current device holds at once of the standalone mean kernel for
+// (N, kind, complex, half-hop): its occupancy per CU times the CUs, cached
+// per device.  Without a device (host-only planning, nothing launched) the
+// kernel's launch bound on 256 CUs stands in.
+static std::mutex g_res_mu;
+static std::map<std::tuple<int, int, int, bool, bool>, int64_t> g_res;
+
+static int64_t resident_blocks(int N, int kk, bool cplx, bool half) {
+  int dev = -1, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1) {
+    (void)hipGetLastError();
+    dev = -1;
+    cus = 256;
+  }
+  const auto key = std::make_tuple(dev, N, kk, cplx, half);
+  {
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    auto it = g_res.find(key);
+    if (it != g_res.end()) return it->second;
+  }
+  int per_cu = 1;
+  switch (N) {
+#define C(NV) \
+  case NV:    \
+    per_cu = mean_blocks_per_cu_##NV(kk, cplx, half); \
+    break;
+    C(16) C(32) C(64) C(128) C(256) C(512) C(1024) C(2048) C(4096)
+#undef C
+  }
+  const int64_t r = (int64_t)(per_cu > 0 ? per_cu : 1) * cus;
+  std::lock_guard<std::mutex> lk(g_res_mu);
+  g_res[key] = r;
+  return r;
+}
+
+// Plan of an FP32 mean job: sized to the resident blocks by default, or to
+// a tuned block target (batch plans under a tuning table).
+static Geo mean_geometry_fp32(const msig_job* j, int maxrun, int64_t target_blocks) {
+  const bool cplx = j->flags & MSIG_F_COMPLEX_INPUT;
+  if (target_blocks > 0) return geometry(j->rows, j->nseg, j->nfft, cplx, maxrun, target_blocks);
+  const int64_t res = resident_blocks(j->nfft, mean_kk(j->kind), cplx, job_half(j));
+  return geometry_resident(j->rows, j->nseg, j->nfft, cplx, maxrun, res);
+}
+
 static Geo mean_geometry(const msig_job* j) {
   if (precision_of(j) != PREC_FP32) return geometry_mfma(j->rows, j->nseg, MEAN_MAX_RUN);
-  const bool cplx = j->flags & MSIG_F_COMPLEX_INPUT;
-  return geometry(j->rows, j->nseg, j->nfft, cplx, MEAN_MAX_RUN);
+  return mean_geometry_fp32(j, MEAN_MAX_RUN, 0);
 }
 
 // Mean-kind workspace: the block partials [rows][bpr][ncomp][nfft] float32.
@@ -768,6 +821,19 @@ int msig_workspace_bytes(const msig_job* job, size_t* bytes) {
   return 0;
 }
 
+int msig_mean_plan(const msig_job* job, int32_t* plan4) {
+  int rc = check_job(job);
+  if (rc) return rc;
+  if (!plan4) return fail(MSIG_EINVAL, "plan4 is NULL");
+  if (!is_mean(job->kind)) return fail(MSIG_EINVAL, "kind %d is not a mean kind", job->kind);
+  const Geo g = mean_geometry(job);
+  plan4[0] = g.units;
+  plan4[1] = g.gpb;
+  plan4[2] = g.run;
+  plan4[3] = g.bpr;
+  return 0;
+}
+
 int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const float* win,
                        void* out0, void* out1, void* out2, void* workspace, size_t ws_bytes,
                        void* stream) {
@@ -790,7 +856,7 @@ int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const 
   if (rc) return rc;
   KArgs a = base_args(job, x, y, win, tw, g);
   a.ws = static_cast<float*>(workspace);
-  const int kk = job->kind == MSIG_PSD_MEAN ? K_PSD : (job->kind == MSIG_CSD_MEAN ? K_CSD : K_COH);
+  const int kk = mean_kk(job->kind);
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = timed_launch(s, [&] {
     return prec != PREC_FP32 ? launch_mean_mfma256(prec, a, job->rows, s)
@@ -964,12 +1030,12 @@ int msig_batch_create(const msig_batch_job* jobs, int32_t njobs, const msig_tuni
     const int N = j->nfft;
     const int l2 = log2i(N);
     int maxrun = MEAN_MAX_RUN;
-    int64_t target = 1024;
+    int64_t target = 0;  // 0: the standalone call's resident-block plan
     if (tuning) {
       if (tuning->max_run[l2] > 0) maxrun = tuning->max_run[l2] < MEAN_MAX_RUN ? tuning->max_run[l2] : MEAN_MAX_RUN;
       if (tuning->target_blocks[l2] > 0) target = tuning->target_blocks[l2];
     }
-    const Geo g = geometry(j->rows, j->nseg, N, cplx, maxrun, target);
+    const Geo g = mean_geometry_fp32(j, maxrun, target);
     const cf* tw = nullptr;
     rc = twiddles(N, &tw);
     if (rc) return rc;

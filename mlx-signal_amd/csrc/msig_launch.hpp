@@ -38,6 +38,38 @@ inline Geo geometry(int64_t rows, int64_t nseg, int N, bool cplx, int maxrun,
 }
 
 constexpr int MEAN_MAX_RUN = 32;    // <= 64 segments summed sequentially in FP32
+
+// Mean-kernel plan sized to the blocks the GPU holds at once (`resident` =
+// blocks per CU at the kernel's occupancy x CUs).  A launch of B blocks runs
+// in ceil(B / resident) rounds of `run` pairs per group, so the plan takes the
+// run (<= maxrun) that minimises rounds x run; among equal costs it keeps
+// the longest run (fewest blocks: fewer partials and block prologues).  A
+// block count a little above the resident count would leave a second round
+// with a quarter of the blocks at low occupancy: 64 x 2^20 at nperseg 1024
+// planned 1024 blocks for 768 resident ones (16 pairs per group, two rounds);
+// this plan gives 768 blocks of 22 pairs per group in one round.
+inline Geo geometry_resident(int64_t rows, int64_t nseg, int N, bool cplx, int maxrun,
+                             int64_t resident) {
+  Geo g;
+  const int tpf = N / elems_per_thread(N);
+  g.gpb = BT / tpf;
+  g.units = (int32_t)(cplx ? nseg : (nseg + 1) / 2);
+  if (resident < 1) resident = 1;
+  int64_t best_cost = -1, best_run = 1;
+  const int64_t top = g.units < maxrun ? g.units : maxrun;
+  for (int64_t run = 1; run <= (top > 1 ? top : 1); ++run) {
+    const int64_t bpr = (g.units + g.gpb * run - 1) / (g.gpb * run);
+    const int64_t rounds = (rows * bpr + resident - 1) / resident;
+    const int64_t cost = rounds * run;
+    if (best_cost < 0 || cost <= best_cost) {
+      best_cost = cost;
+      best_run = run;
+    }
+  }
+  g.run = (int32_t)best_run;
+  g.bpr = (int32_t)((g.units + (int64_t)g.gpb * best_run - 1) / ((int64_t)g.gpb * best_run));
+  return g;
+}
 constexpr int FRAMES_MAX_RUN = 8;
 // Per-frame kernels: about 1536 blocks (stft 16 x 2^20, nperseg 1024, ran in
 // 52 us at this target against 59 us at 1024; the power frames were flat).
@@ -78,7 +110,8 @@ hipError_t launch_mean_mfma256(int prec, const KArgs& a, int64_t rows, hipStream
                                     const KArgs* jobs, const WorkItem* queue, int32_t nwork, \
                                     float* ws, int grid, hipStream_t s);                    \
   hipError_t launch_istft_##NV(const KArgs& a, int64_t rows, hipStream_t s);                 \
-  hipError_t launch_istft_ola_##NV(const KArgs& a, int64_t rows, hipStream_t s);
+  hipError_t launch_istft_ola_##NV(const KArgs& a, int64_t rows, hipStream_t s);             \
+  int mean_blocks_per_cu_##NV(int kind, bool cplx, bool half);
 MSIG_DECLARE_LAUNCHERS(16)
 MSIG_DECLARE_LAUNCHERS(32)
 MSIG_DECLARE_LAUNCHERS(64)
@@ -124,8 +157,30 @@ MSIG_DECLARE_LAUNCHERS(4096)
   else if (kind == K1) L(NV, K1, C, H);           \
   else L(NV, K2, C, H);
 
+// Blocks per CU of one standalone mean kernel: the runtime's occupancy of the
+// compiled kernel, or (no device, host-only planning) its launch bound.
+#define MSIG_MEAN_OCC(NV, K, C, H)                                                        \
+  do {                                                                                    \
+    int nb = 0;                                                                           \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                     \
+            &nb, reinterpret_cast<const void*>(&k_mean<NV, (NV < 16 ? NV : 16), K, C, H>), \
+            BT, 0) == hipSuccess && nb > 0)                                               \
+      return nb;                                                                          \
+    (void)hipGetLastError();                                                              \
+    return mean_waves<NV, K, H, C>();                                                     \
+  } while (0)
+
 #define MSIG_DEFINE_LAUNCHERS(NV)                                                          \
   namespace msig {                                                                         \
+  int mean_blocks_per_cu_##NV(int kind, bool cplx, bool half) {                            \
+    if (cplx) {                                                                            \
+      MSIG_KIND_SWITCH(MSIG_MEAN_OCC, NV, K_PSD, K_CSD, K_COH, true, false)                \
+    } else if (half) {                                                                     \
+      MSIG_KIND_SWITCH(MSIG_MEAN_OCC, NV, K_PSD, K_CSD, K_COH, false, true)                \
+    } else {                                                                               \
+      MSIG_KIND_SWITCH(MSIG_MEAN_OCC, NV, K_PSD, K_CSD, K_COH, false, false)               \
+    }                                                                                      \
+  }                                                                                        \
   hipError_t launch_mean_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows, \
                               hipStream_t s) {                                             \
     const dim3 grid((unsigned)(rows * a.bpr)), block(BT);                                  \

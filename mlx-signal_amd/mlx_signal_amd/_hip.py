@@ -17,7 +17,7 @@ import os
 
 # MSIG_LIBRARY selects another build of the library (e.g. a tuning variant)
 LIB_PATH = Path(os.environ.get("MSIG_LIBRARY") or Path(__file__).with_name("libmsig.so"))
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 MSIG_EINVAL = -22
 MSIG_ENOTSUP = -95
@@ -109,6 +109,7 @@ SIGNATURES = {
     "msig_init": (ctypes.c_int, [ctypes.c_int]),
     "msig_eligible": (ctypes.c_int, [_i32, _i32, _i32, _i32, _i32]),
     "msig_workspace_bytes": (ctypes.c_int, [_JOBP, ctypes.POINTER(_sz)]),
+    "msig_mean_plan": (ctypes.c_int, [_JOBP, ctypes.POINTER(_i32)]),
     "msig_spectral_mean": (ctypes.c_int, [_JOBP, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "msig_spectral_frames": (ctypes.c_int, [_JOBP, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "msig_rfft_frames": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i32, _i32, _i32, _i32, _vp, _vp]),

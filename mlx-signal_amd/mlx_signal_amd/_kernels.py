@@ -122,6 +122,22 @@ def mfma_eligible(kind: str, x2, nperseg: int, nfft: int) -> bool:
             and nfft == _hip.MFMA_NFFT)
 
 
+def mean_geometry(kind: str, rows: int, n: int, nperseg: int, hop: int, nfft: int,
+                  detrend=False, cplx: bool = False) -> dict:
+    """The launch plan the library picks for a mean job on the current
+    device: units per row, FFT groups per block, units per group (run) and
+    blocks per row.  Diagnostic (msig_mean_plan)."""
+    lib = _hip.load()
+    flags = _hip.F_ONESIDED | (_hip.F_COMPLEX_INPUT if cplx else 0)
+    flags |= _hip.F_DETREND_CONST if _is_const(detrend) else 0
+    nseg = (n - nperseg) // hop + 1
+    job = _hip.MsigJob(_MEAN_KINDS[kind], flags, rows, n, n, n, nperseg, hop, nfft, nseg, 1.0,
+                       0, 0)
+    out = (ctypes.c_int32 * 4)()
+    _hip.check(lib.msig_mean_plan(ctypes.byref(job), out), "mean_plan")
+    return {"units": out[0], "gpb": out[1], "run": out[2], "bpr": out[3]}
+
+
 # Workspace bytes per (kind, flags, rows, nseg, nfft): the block partials'
 # size is a function of the launch geometry, which depends on nothing else.
 _WS_BYTES: dict = {}
