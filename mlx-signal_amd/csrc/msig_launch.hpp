@@ -40,14 +40,16 @@ inline Geo geometry(int64_t rows, int64_t nseg, int N, bool cplx, int maxrun,
 constexpr int MEAN_MAX_RUN = 32;    // <= 64 segments summed sequentially in FP32
 
 // Mean-kernel plan sized to the blocks the GPU holds at once (`resident` =
-// blocks per CU at the kernel's occupancy x CUs).  A launch of B blocks runs
-// in ceil(B / resident) rounds of `run` pairs per group, so the plan takes the
-// run (<= maxrun) that minimises rounds x run; among equal costs it keeps
-// the longest run (fewest blocks: fewer partials and block prologues).  A
-// block count a little above the resident count would leave a second round
-// with a quarter of the blocks at low occupancy: 64 x 2^20 at nperseg 1024
-// planned 1024 blocks for 768 resident ones (16 pairs per group, two rounds);
-// this plan gives 768 blocks of 22 pairs per group in one round.
+// blocks per CU at the kernel's occupancy x CUs).  Blocks of equal work
+// launched B at a time run in ceil(B / resident) rounds of `run` pairs per
+// group, so the cost of a run is rounds x run.  The plan keeps the longest
+// run (maxrun: fewest blocks, partials and block prologues) unless a shorter
+// one costs at least 5% less; among equal costs the longer run wins.  A block
+// count a little above the resident count would leave a second round with a
+// quarter of the blocks at low occupancy: 64 x 2^20 at nperseg 1024 planned
+// 1024 blocks for 768 resident ones (16 pairs per group, two rounds); this
+// plan gives 768 blocks of 22 pairs per group in one round.  Jobs of many
+// rounds keep maxrun, so a row of a large call is planned as it is alone.
 inline Geo geometry_resident(int64_t rows, int64_t nseg, int N, bool cplx, int maxrun,
                              int64_t resident) {
   Geo g;
@@ -55,21 +57,25 @@ inline Geo geometry_resident(int64_t rows, int64_t nseg, int N, bool cplx, int m
   g.gpb = BT / tpf;
   g.units = (int32_t)(cplx ? nseg : (nseg + 1) / 2);
   if (resident < 1) resident = 1;
-  int64_t best_cost = -1, best_run = 1;
-  const int64_t top = g.units < maxrun ? g.units : maxrun;
-  for (int64_t run = 1; run <= (top > 1 ? top : 1); ++run) {
+  auto cost_of = [&](int64_t run) {
     const int64_t bpr = (g.units + g.gpb * run - 1) / (g.gpb * run);
-    const int64_t rounds = (rows * bpr + resident - 1) / resident;
-    const int64_t cost = rounds * run;
-    if (best_cost < 0 || cost <= best_cost) {
+    return ((rows * bpr + resident - 1) / resident) * run;
+  };
+  const int64_t top = g.units < maxrun ? (g.units > 1 ? g.units : 1) : maxrun;
+  int64_t best_cost = cost_of(top), best_run = top;
+  for (int64_t run = top - 1; run >= 1; --run) {
+    const int64_t cost = cost_of(run);
+    if (cost < best_cost) {
       best_cost = cost;
       best_run = run;
     }
   }
+  if (20 * best_cost > 19 * cost_of(top)) best_run = top;
   g.run = (int32_t)best_run;
   g.bpr = (int32_t)((g.units + (int64_t)g.gpb * best_run - 1) / ((int64_t)g.gpb * best_run));
   return g;
 }
+
 constexpr int FRAMES_MAX_RUN = 8;
 // Per-frame kernels: about 1536 blocks (stft 16 x 2^20, nperseg 1024, ran in
 // 52 us at this target against 59 us at 1024; the power frames were flat).

@@ -18,6 +18,10 @@ def test_welch_rows_past_2_31_elements():
     g.manual_seed(3)
     n = 1 << 30  # 3 rows: row 2 starts at element 2^31
     x = torch.randn(3, n, device=dev, generator=g)
+    from mlx_signal_amd import _kernels
+
+    assert (_kernels.mean_geometry("psd", 3, n, 1024, 512, 1024)
+            == _kernels.mean_geometry("psd", 1, n, 1024, 512, 1024))
     _, p = msig.welch(x, nperseg=1024)
     for r in (0, 2):
         _, p_row = msig.welch(x[r:r + 1].clone(), nperseg=1024)
