@@ -142,6 +142,26 @@ int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const 
 int msig_spectral_frames(const msig_job* job, const void* x, const void* y, const float* win,
                          void* out0, void* out1, void* out2, void* stream);
 
+/* Largest block-partial workspace (bytes) msig_spectral_mean_noalloc takes. */
+#define MSIG_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+
+/* msig_spectral_mean without a workspace argument and without any device
+ * allocation: the block partials go to a statically allocated 16 MiB array
+ * in the current device's memory (held until the process ends), so that
+ * callers which cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation) can run the fused means.
+ * No counterpart in the reference.  Set-up: msig_init(device) must have run
+ * once on the device (it resolves the array's device address and builds the
+ * resident tables); otherwise MSIG_EINVAL.  Jobs whose partials
+ * (msig_workspace_bytes) exceed MSIG_NOALLOC_SCRATCH_BYTES get MSIG_EINVAL.
+ * Every call on one device shares the array, so calls that overlap in time
+ * on one device (several streams, several threads) race; calls in one
+ * stream's order are safe.  Same kernels and launch plan as
+ * msig_spectral_mean, so the outputs are bit-identical. */
+int msig_spectral_mean_noalloc(const msig_job* job, const void* x, const void* y,
+                               const float* win, void* out0, void* out1, void* out2,
+                               void* stream);
+
 /* ---- operator-level entry points: one per function of the reference's fused
  * module.  Real float32 input, rows contiguous (ld = n), onesided, nfft ==
  * nperseg, `win` carries any scale (as in the reference), detrend is 0 (none)

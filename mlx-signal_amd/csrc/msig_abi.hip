@@ -75,6 +75,48 @@ static int twiddles(int N, const cf** out) {
   return 0;
 }
 
+// Table of (device, N) if it exists; never allocates.
+static const cf* resident_twiddles(int dev, int N) {
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto it = g_tw.find({dev, N});
+  return it == g_tw.end() ? nullptr : it->second;
+}
+
+// ---------------------------------------------------------------- no-allocation scratch
+// Block partials of msig_spectral_mean_noalloc: one statically allocated
+// array in each device's memory, held until the process ends.  In host code
+// `scratch` is only a stand-in symbol; its address on a device comes from
+// hipGetSymbolAddress with that device current (msig_init), and only that
+// address is ever passed to kernels.  Every call on a device shares its
+// array: calls that overlap in time on one device race.  The mean kernel
+// writes every partial it later reads, so nothing needs clearing.
+static __device__ __align__(256) float scratch[MSIG_NOALLOC_SCRATCH_BYTES / sizeof(float)];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scr_mu;
+static float* g_scr[SCRATCH_MAX_DEVICES] = {};
+
+// with `dev` current
+static int scratch_setup(int dev) {
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+    return fail(MSIG_EINVAL, "device %d beyond the %d devices the scratch table covers", dev,
+                SCRATCH_MAX_DEVICES);
+  std::lock_guard<std::mutex> lk(g_scr_mu);
+  if (g_scr[dev]) return 0;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  if (!p) return fail(MSIG_EIO, "hipGetSymbolAddress(scratch) returned NULL");
+  g_scr[dev] = static_cast<float*>(p);
+  return 0;
+}
+
+static float* scratch_of(int dev) {
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scr_mu);
+  return g_scr[dev];
+}
+
 // ---------------------------------------------------------------- kernel timing
 // Event pairs handed out to the next calls on the armed device, one pair per
 // fused transform kernel.  The vector is sized at arm time and never grows
@@ -799,8 +841,14 @@ int msig_init(int device) {
     const cf* t = nullptr;
     int rc = twiddles(N, &t);
     if (rc) return rc;
+    // every mean kernel's resident-block count, so later plans only look up
+    for (int kk = K_PSD; kk <= K_COH; ++kk) {
+      (void)resident_blocks(N, kk, true, false);
+      (void)resident_blocks(N, kk, false, false);
+      (void)resident_blocks(N, kk, false, true);
+    }
   }
-  return 0;
+  return scratch_setup(device);
 }
 
 int msig_eligible(int32_t nperseg, int32_t nfft, int32_t onesided, int32_t detrend_const,
@@ -834,9 +882,9 @@ int msig_mean_plan(const msig_job* job, int32_t* plan4) {
   return 0;
 }
 
-int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const float* win,
-                       void* out0, void* out1, void* out2, void* workspace, size_t ws_bytes,
-                       void* stream) {
+// Checks shared by the mean entry points.
+static int check_mean_call(const msig_job* job, const void* x, const void* y, const float* win,
+                           void* out0, void* out1, void* out2) {
   int rc = check_job(job);
   if (rc) return rc;
   if (!is_mean(job->kind)) return fail(MSIG_EINVAL, "kind %d is not a mean kind", job->kind);
@@ -844,20 +892,21 @@ int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const 
   if (!x || !win || !out0 || (pair && !y)) return fail(MSIG_EINVAL, "NULL input/output pointer");
   if (job->kind == MSIG_COH_TRIO_MEAN && (!out1 || !out2))
     return fail(MSIG_EINVAL, "coherence trio needs three outputs");
-  const size_t need = mean_ws_bytes(job);
-  if (!workspace || ws_bytes < need)
-    return fail(MSIG_EINVAL, "workspace of %zu bytes needed (got %zu)", need, ws_bytes);
+  return 0;
+}
+
+// The two kernels of a checked mean job, block partials in `ws` (which holds
+// at least mean_ws_bytes(job)); `tw` is the size's resident twiddle table.
+static int mean_run(const msig_job* job, const void* x, const void* y, const float* win,
+                    void* out0, void* out1, void* out2, float* ws, const cf* tw,
+                    hipStream_t s) {
   const bool cplx = job->flags & MSIG_F_COMPLEX_INPUT;
   const int N = job->nfft;
   const int prec = precision_of(job);
   const Geo g = mean_geometry(job);
-  const cf* tw = nullptr;
-  rc = twiddles(N, &tw);
-  if (rc) return rc;
   KArgs a = base_args(job, x, y, win, tw, g);
-  a.ws = static_cast<float*>(workspace);
+  a.ws = ws;
   const int kk = mean_kk(job->kind);
-  hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = timed_launch(s, [&] {
     return prec != PREC_FP32 ? launch_mean_mfma256(prec, a, job->rows, s)
                              : dispatch_mean(N, kk, cplx, a, job->rows, s);
@@ -888,6 +937,46 @@ int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const 
   e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "finalize kernel launch");
   return 0;
+}
+
+int msig_spectral_mean(const msig_job* job, const void* x, const void* y, const float* win,
+                       void* out0, void* out1, void* out2, void* workspace, size_t ws_bytes,
+                       void* stream) {
+  int rc = check_mean_call(job, x, y, win, out0, out1, out2);
+  if (rc) return rc;
+  const size_t need = mean_ws_bytes(job);
+  if (!workspace || ws_bytes < need)
+    return fail(MSIG_EINVAL, "workspace of %zu bytes needed (got %zu)", need, ws_bytes);
+  const cf* tw = nullptr;
+  rc = twiddles(job->nfft, &tw);
+  if (rc) return rc;
+  return mean_run(job, x, y, win, out0, out1, out2, static_cast<float*>(workspace), tw,
+                  static_cast<hipStream_t>(stream));
+}
+
+int msig_spectral_mean_noalloc(const msig_job* job, const void* x, const void* y,
+                               const float* win, void* out0, void* out1, void* out2,
+                               void* stream) {
+  int rc = check_mean_call(job, x, y, win, out0, out1, out2);
+  if (rc) return rc;
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  float* ws = scratch_of(dev);
+  if (!ws)
+    return fail(MSIG_EINVAL,
+                "msig_init(%d) must run once on this device before msig_spectral_mean_noalloc "
+                "(it resolves the device's scratch array and builds the resident tables)",
+                dev);
+  const size_t need = mean_ws_bytes(job);
+  if (need > MSIG_NOALLOC_SCRATCH_BYTES)
+    return fail(MSIG_EINVAL,
+                "block partials of %zu bytes exceed the %zu-byte scratch of "
+                "msig_spectral_mean_noalloc (use msig_spectral_mean with a workspace)",
+                need, (size_t)MSIG_NOALLOC_SCRATCH_BYTES);
+  const cf* tw = resident_twiddles(dev, job->nfft);
+  if (!tw) return fail(MSIG_EINVAL, "no resident twiddle table for nfft %d on device %d", job->nfft, dev);
+  return mean_run(job, x, y, win, out0, out1, out2, ws, tw, static_cast<hipStream_t>(stream));
 }
 
 int msig_spectral_frames(const msig_job* job, const void* x, const void* y, const float* win,

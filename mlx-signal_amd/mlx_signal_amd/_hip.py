@@ -25,6 +25,7 @@ MSIG_EIO = -5
 
 MIN_NFFT = 16
 MAX_NFFT = 4096
+NOALLOC_SCRATCH_BYTES = 16 << 20  # MSIG_NOALLOC_SCRATCH_BYTES
 
 # kinds
 PSD_MEAN, CSD_MEAN, COH_TRIO_MEAN, COHERENCE = 0, 1, 2, 3
@@ -112,6 +113,7 @@ SIGNATURES = {
     "msig_mean_plan": (ctypes.c_int, [_JOBP, ctypes.POINTER(_i32)]),
     "msig_spectral_mean": (ctypes.c_int, [_JOBP, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "msig_spectral_frames": (ctypes.c_int, [_JOBP, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "msig_spectral_mean_noalloc": (ctypes.c_int, [_JOBP, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "msig_rfft_frames": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i32, _i32, _i32, _i32, _vp, _vp]),
     "msig_rfft_power_mean": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i32, _i32, _i32, _vp, _vp, _sz, _vp]),
     "msig_csd_frames": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _i32, _i32, _i32, _vp, _vp]),

@@ -190,3 +190,10 @@ def test_side_copy_arguments_checked_without_gpu():
     assert lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(sz)) == _hip.MSIG_EINVAL
     p = ctypes.c_void_p(16)
     assert lib.msig_istft_ola(p, 1, 4, 256, p, p, 0, 10, p, p, None, 4, None) == _hip.MSIG_EINVAL
+
+
+def test_noalloc_scratch_limit_matches_header():
+    text = (Path(__file__).resolve().parents[1] / "include" / "msig.h").read_text()
+    m = re.search(r"#define MSIG_NOALLOC_SCRATCH_BYTES \(\(size_t\)(\d+) << (\d+)\)", text)
+    assert m, "MSIG_NOALLOC_SCRATCH_BYTES not found"
+    assert int(m.group(1)) << int(m.group(2)) == _hip.NOALLOC_SCRATCH_BYTES
