@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -42,44 +43,73 @@ static int hip_fail(hipError_t e, const char* what) {
   return fail(MSIG_EIO, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
 }
 
-// ---------------------------------------------------------------- twiddles
-// exp(-2 pi i e / N), e < N, computed in FP64 and rounded once to float.
-static std::mutex g_tw_mu;
-static std::map<std::pair<int, int>, cf*> g_tw;  // (device, N) -> device table
+// ---------------------------------------------------------------- per-device tables
+// What the library keeps per device, looked up without locks on the hot
+// path: the twiddle table of each fused size, the resident-block count of
+// each standalone mean kernel, and the CU count.  Entries are written once
+// (under g_dev_mu) and read atomically.
+constexpr int MAX_DEVICES = 64;
+constexpr int NSIZES = 13;  // log2(nfft) <= 12
+struct DeviceTables {
+  std::atomic<const cf*> tw[NSIZES];
+  std::atomic<int64_t> resident[NSIZES][3][3];  // [log2 N][kind][complex, general, half-hop]
+  std::atomic<int> cus;
+};
+static DeviceTables g_dev[MAX_DEVICES];
+static std::mutex g_dev_mu;
+
+static int log2i(int v);
+
+static int current_device(int* dev) {
+  hipError_t e = hipGetDevice(dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (*dev < 0 || *dev >= MAX_DEVICES)
+    return fail(MSIG_EINVAL, "device %d beyond the %d devices the library tracks", *dev, MAX_DEVICES);
+  return 0;
+}
+
+// exp(-2 pi i e / N), e < N, computed in FP64 and rounded once to float;
+// built on first use per (device, N) and kept until the process ends.
+static int twiddles_on(int dev, int N, const cf** out) {
+  std::atomic<const cf*>& slot = g_dev[dev].tw[log2i(N)];
+  const cf* t = slot.load(std::memory_order_acquire);
+  if (t) {
+    *out = t;
+    return 0;
+  }
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  t = slot.load(std::memory_order_acquire);
+  if (!t) {
+    std::vector<cf> h(N);
+    for (int i = 0; i < N; ++i) {
+      const double ang = -2.0 * M_PI * (double)i / (double)N;
+      h[i] = cf{(float)std::cos(ang), (float)std::sin(ang)};
+    }
+    cf* d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof(cf) * N);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(twiddles)");
+    e = hipMemcpy(d, h.data(), sizeof(cf) * N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      return hip_fail(e, "hipMemcpy(twiddles)");
+    }
+    slot.store(d, std::memory_order_release);
+    t = d;
+  }
+  *out = t;
+  return 0;
+}
 
 static int twiddles(int N, const cf** out) {
   int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-  std::lock_guard<std::mutex> lk(g_tw_mu);
-  auto it = g_tw.find({dev, N});
-  if (it != g_tw.end()) {
-    *out = it->second;
-    return 0;
-  }
-  std::vector<cf> h(N);
-  for (int i = 0; i < N; ++i) {
-    const double ang = -2.0 * M_PI * (double)i / (double)N;
-    h[i] = cf{(float)std::cos(ang), (float)std::sin(ang)};
-  }
-  cf* d = nullptr;
-  e = hipMalloc(&d, sizeof(cf) * N);
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc(twiddles)");
-  e = hipMemcpy(d, h.data(), sizeof(cf) * N, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    (void)hipFree(d);
-    return hip_fail(e, "hipMemcpy(twiddles)");
-  }
-  g_tw[{dev, N}] = d;
-  *out = d;
-  return 0;
+  int rc = current_device(&dev);
+  return rc ? rc : twiddles_on(dev, N, out);
 }
 
 // Table of (device, N) if it exists; never allocates.
 static const cf* resident_twiddles(int dev, int N) {
-  std::lock_guard<std::mutex> lk(g_tw_mu);
-  auto it = g_tw.find({dev, N});
-  return it == g_tw.end() ? nullptr : it->second;
+  if (dev < 0 || dev >= MAX_DEVICES) return nullptr;
+  return g_dev[dev].tw[log2i(N)].load(std::memory_order_acquire);
 }
 
 // ---------------------------------------------------------------- no-allocation scratch
@@ -128,8 +158,10 @@ struct TimingEvents {
 };
 static std::mutex g_kt_mu;
 static TimingEvents g_kt;
+static std::atomic<bool> g_kt_armed{false};  // skip the lock when nothing is armed
 
 static void timing_release() {
+  g_kt_armed.store(false, std::memory_order_release);
   for (auto& p : g_kt.ev) {
     (void)hipEventDestroy(p.first);
     (void)hipEventDestroy(p.second);
@@ -140,6 +172,7 @@ static void timing_release() {
 }
 
 static std::pair<hipEvent_t, hipEvent_t>* timing_slot() {
+  if (!g_kt_armed.load(std::memory_order_acquire)) return nullptr;
   std::lock_guard<std::mutex> lk(g_kt_mu);
   if (g_kt.used >= g_kt.ev.size()) return nullptr;
   int dev = -1;
@@ -711,23 +744,30 @@ static bool job_half(const msig_job* j) {
 // (N, kind, complex, half-hop): its occupancy per CU times the CUs, cached
 // per device.  Without a device (host-only planning, nothing launched) the
 // kernel's launch bound on 256 CUs stands in.
-static std::mutex g_res_mu;
-static std::map<std::tuple<int, int, int, bool, bool>, int64_t> g_res;
-
 static int64_t resident_blocks(int N, int kk, bool cplx, bool half) {
-  int dev = -1, cus = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      cus < 1) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
     (void)hipGetLastError();
     dev = -1;
-    cus = 256;
   }
-  const auto key = std::make_tuple(dev, N, kk, cplx, half);
-  {
-    std::lock_guard<std::mutex> lk(g_res_mu);
-    auto it = g_res.find(key);
-    if (it != g_res.end()) return it->second;
+  const int var = cplx ? 0 : (half ? 2 : 1);
+  std::atomic<int64_t>* slot = dev >= 0 ? &g_dev[dev].resident[log2i(N)][kk][var] : nullptr;
+  if (slot) {
+    const int64_t r = slot->load(std::memory_order_acquire);
+    if (r > 0) return r;
+  }
+  int cus = 256;
+  if (dev >= 0) {
+    cus = g_dev[dev].cus.load(std::memory_order_acquire);
+    if (cus <= 0) {
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          cus < 1) {
+        (void)hipGetLastError();
+        cus = 256;
+      } else {
+        g_dev[dev].cus.store(cus, std::memory_order_release);
+      }
+    }
   }
   int per_cu = 1;
   switch (N) {
@@ -739,8 +779,7 @@ static int64_t resident_blocks(int N, int kk, bool cplx, bool half) {
 #undef C
   }
   const int64_t r = (int64_t)(per_cu > 0 ? per_cu : 1) * cus;
-  std::lock_guard<std::mutex> lk(g_res_mu);
-  g_res[key] = r;
+  if (slot) slot->store(r, std::memory_order_release);
   return r;
 }
 
@@ -1535,6 +1574,7 @@ int msig_kernel_timing_arm(int32_t max_launches) {
     g_kt.ev.emplace_back(a, b);
   }
   g_kt.device = dev;
+  g_kt_armed.store(true, std::memory_order_release);
   return 0;
 }
 

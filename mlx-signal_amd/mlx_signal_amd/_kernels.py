@@ -66,21 +66,32 @@ def eligible(x, nperseg: int, nfft: int, sides: str, detrend) -> bool:
 _WORKSPACES: dict = {}
 
 
-def _workspace(dev: torch.device, nbytes: int) -> torch.Tensor:
-    """Block-partials buffer, cached per (device, current stream): calls on one
-    stream run in order, so they can share it; other streams get their own."""
-    key = (dev.index, _hip.stream_of_device(dev))
+def workspace_and_stream(dev: torch.device, nbytes: int):
+    """(block-partials buffer, raw handle of the current stream).  The buffer
+    is cached per (device, current stream): calls on one stream run in order,
+    so they can share it; other streams get their own."""
+    stream = _hip.stream_of_device(dev)
+    key = (dev.index, stream)
     buf = _WORKSPACES.get(key)
     if buf is None or buf.numel() < nbytes:
         buf = torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, device=dev)
         _WORKSPACES[key] = buf
-    return buf
+    return buf, stream
 
 
-def _rows(t: torch.Tensor) -> torch.Tensor:
+def _workspace(dev: torch.device, nbytes: int) -> torch.Tensor:
+    return workspace_and_stream(dev, nbytes)[0]
+
+
+def row_major(t: torch.Tensor) -> torch.Tensor:
+    """A (rows, n) tensor whose rows are contiguous and do not overlap (a
+    copy only when the view is not)."""
     if t.stride(-1) != 1 or (t.dim() == 2 and t.stride(0) < t.shape[1]):
         t = t.contiguous()
     return t
+
+
+_rows = row_major
 
 
 def _job(kind, x2, y2, nperseg, nstep, nfft, detrend, onesided, apply_factor, scale,
@@ -157,6 +168,32 @@ def _side(side, dev):
     return src.data_ptr(), dst.data_ptr(), src.numel()
 
 
+def mean_job(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, onesided=True,
+             apply_factor=False, scale=1.0, precision="fp32"):
+    """(job descriptor, workspace bytes, launch-counter name) of a fused mean
+    over row-major (rows, n) inputs; checks the arguments as spectral_mean."""
+    _check_inputs(x2, y2, win, nperseg)
+    code = _MEAN_KINDS[kind]
+    if code != _hip.PSD_MEAN and y2 is None:
+        raise ValueError(f"{kind} needs two signals")
+    if precision != "fp32" and precision not in _PRECISION_FLAGS:
+        raise ValueError(f"unknown precision {precision!r}")
+    job = _job(code, x2, y2, nperseg, nstep, nfft, detrend, onesided, apply_factor, scale)
+    mfma = precision != "fp32" and mfma_eligible(kind, x2, nperseg, nfft)
+    if mfma:
+        job.flags |= _PRECISION_FLAGS[precision]
+    key = (job.kind, job.flags, job.rows, job.nseg, job.nfft)  # what the partials depend on
+    nbytes = _WS_BYTES.get(key)
+    if nbytes is None:
+        ws_bytes = ctypes.c_size_t(0)
+        _hip.check(_hip.load().msig_workspace_bytes(ctypes.byref(job), ctypes.byref(ws_bytes)),
+                   kind)
+        nbytes = _WS_BYTES[key] = ws_bytes.value
+        if len(_WS_BYTES) > 4096:
+            _WS_BYTES.clear()
+    return job, nbytes, "mean_" + kind + ("_" + precision if mfma else "")
+
+
 def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, onesided=True,
                   apply_factor=False, scale=1.0, precision="fp32", side=None):
     """Mean over segments of |X|^2 ("psd"), conj(X) Y ("csd"), the
@@ -172,29 +209,14 @@ def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, oneside
     :func:`mfma_eligible` holds and the FP32 kernels elsewhere."""
     x2 = _rows(x2)
     y2 = None if y2 is None else _rows(y2)
-    _check_inputs(x2, y2, win, nperseg)
     code = _MEAN_KINDS[kind]
-    if code != _hip.PSD_MEAN and y2 is None:
-        raise ValueError(f"{kind} needs two signals")
-    if precision != "fp32" and precision not in _PRECISION_FLAGS:
-        raise ValueError(f"unknown precision {precision!r}")
+    job, nbytes, counter = mean_job(kind, x2, y2, win, nperseg, nstep, nfft, detrend, onesided,
+                                    apply_factor, scale, precision)
     lib = _hip.load()
-    job = _job(code, x2, y2, nperseg, nstep, nfft, detrend, onesided, apply_factor, scale)
-    mfma = precision != "fp32" and mfma_eligible(kind, x2, nperseg, nfft)
-    if mfma:
-        job.flags |= _PRECISION_FLAGS[precision]
     cplx = x2.dtype == torch.complex64
     bins = nfft // 2 + 1 if (onesided and not cplx) else nfft
     rows = x2.shape[0]
     dev = x2.device
-    key = (job.kind, job.flags, job.rows, job.nseg, job.nfft)  # what the partials depend on
-    nbytes = _WS_BYTES.get(key)
-    if nbytes is None:
-        ws_bytes = ctypes.c_size_t(0)
-        _hip.check(lib.msig_workspace_bytes(ctypes.byref(job), ctypes.byref(ws_bytes)), kind)
-        nbytes = _WS_BYTES[key] = ws_bytes.value
-        if len(_WS_BYTES) > 4096:
-            _WS_BYTES.clear()
     ws = _workspace(dev, nbytes)
     f32 = dict(dtype=torch.float32, device=dev)
     c64 = dict(dtype=torch.complex64, device=dev)
@@ -211,7 +233,7 @@ def spectral_mean(kind: str, x2, y2, win, nperseg, nstep, nfft, detrend, oneside
                                 _hip.ptr(o[0]), _hip.ptr(o[1]), _hip.ptr(o[2]), _hip.ptr(ws),
                                 ws.numel(), _hip.stream_of(x2))
     _hip.check(rc, f"spectral_mean({kind})")
-    launches["mean_" + kind + ("_" + precision if mfma else "")] += 1
+    launches[counter] += 1
     return outs
 
 

@@ -25,9 +25,8 @@ import dataclasses
 
 import torch
 
-from . import _hip
+from . import _hip, _plan
 from ._device import compute_device
-from .spectral import _freqs, _triage_segments, _window_dev
 
 __all__ = ["SpectralRequest", "SpectralBatch", "tuning_struct"]
 
@@ -70,8 +69,7 @@ _KINDS = {"welch": _hip.PSD_MEAN, "csd": _hip.CSD_MEAN, "coherence": _hip.COHERE
 class SpectralBatch:
     """A planned ragged batch.  ``run()`` launches it on the current stream;
     ``outputs[i]`` is request i's result tensor ((..., bins)), rewritten by
-    every run; ``freqs[i]`` its frequency grid, the device-cached constant
-    shared with other requests and calls (read-only: clone it to modify)."""
+    every run; ``freqs[i]`` its frequency grid (the batch's own copy)."""
 
     def __init__(self, requests, tuning=None):
         self.device = compute_device()
@@ -82,7 +80,8 @@ class SpectralBatch:
         jobs = (_hip.MsigBatchJob * n)()
         self._keep = []
         self.outputs = []
-        self.freqs = []
+        self._grids = []  # cached device grids, shared with other calls
+        self._freqs = None
         self.samples = 0
         self.bytes = 0
         for i, r in enumerate(requests):
@@ -121,14 +120,8 @@ class SpectralBatch:
         batch_shape = tuple(x.shape[:-1])
         x2 = x.reshape(-1, n).contiguous()
         y2 = y.reshape(-1, n).contiguous() if pair else None
-        win_np, nperseg = _triage_segments(r.window, r.nperseg, input_length=n)
-        nfft = nperseg if r.nfft is None else int(r.nfft)
-        if nfft < nperseg:
-            raise ValueError("nfft must be greater than or equal to nperseg.")
-        noverlap = nperseg // 2 if r.noverlap is None else int(r.noverlap)
-        if noverlap >= nperseg:
-            raise ValueError("noverlap must be less than nperseg.")
-        nstep = nperseg - noverlap
+        win_np, nperseg = _plan.resolve_window(r.window, _plan.check_nperseg(r.nperseg), n)
+        _, nstep, nfft = _plan.resolve_lengths(nperseg, r.noverlap, r.nfft)
         detrend = r.detrend
         const = isinstance(detrend, str) and detrend in ("constant", "c")
         if not (const or detrend is False or detrend is None):
@@ -138,22 +131,17 @@ class SpectralBatch:
             raise ValueError(f"nfft={nfft} is outside the fused power-of-two range")
         if r.kind == "coherence":
             scale, onesided = 1.0, True
-        elif r.scaling == "density":
-            scale = 1.0 / (r.fs * float((win_np * win_np).sum()))
-        elif r.scaling == "spectrum":
-            scale = 1.0 / float(win_np.sum()) ** 2
         else:
-            raise ValueError(f"Unknown scaling: {r.scaling!r}")
+            scale = _plan.spectral_scale(r.scaling, r.fs, win_np)
         bins = nfft // 2 + 1 if onesided else nfft
         rows = x2.shape[0]
-        nseg = (n - nperseg) // nstep + 1
+        nseg = _plan.segment_count(n, nperseg, nstep)
         dtype = torch.complex64 if r.kind == "csd" else torch.float32
         out = torch.empty(rows, bins, dtype=dtype, device=self.device)
-        win = _window_dev(r.window, win_np, self.device)
+        win = _plan.window_on(self.device, r.window, win_np)
         self._keep += [x2, y2, win, out]
         self.outputs.append(out.reshape(*batch_shape, bins))
-        # shared, read-only frequency grid (cached on the device per nfft / fs)
-        self.freqs.append(_freqs(nfft, r.fs, "onesided" if onesided else "twosided", self.device))
+        self._grids.append(_plan.freqs_on(self.device, nfft, r.fs, onesided))
         self.samples += rows * n
         self.bytes += (2 if pair else 1) * 4 * rows * n + out.element_size() * rows * bins
         flags = (_hip.F_ONESIDED if onesided else 0) | (_hip.F_DETREND_CONST if const else 0)
@@ -163,6 +151,14 @@ class SpectralBatch:
                            y2.stride(0) if pair else n, nperseg, nstep, nfft, nseg, scale)
         return _hip.MsigBatchJob(job, x2.data_ptr(), y2.data_ptr() if pair else None,
                                  win.data_ptr(), out.data_ptr(), None, None)
+
+    @property
+    def freqs(self):
+        """Each request's frequency grid, as tensors this batch owns (copies
+        of the shared cached grids, made on first access)."""
+        if self._freqs is None:
+            self._freqs = [g.clone() for g in self._grids]
+        return self._freqs
 
     def run(self):
         """Launch the batch on the current stream of its device."""

@@ -1,39 +1,50 @@
-"""Window functions: a cached wrapper over scipy.signal.get_window.
+"""Windows: scipy.signal.get_window's values, kept in float64 on the host.
 
-As in the reference (src/mlx_signal_processing/windows.py:19-35) the window
-values are computed on the host in float64 (so sum(w) and sum(w**2) keep full
-precision) and handed to the device as float32.
+The spectral scale factors (sum w, sum w^2) are computed from the float64
+values; the kernels read a float32 copy on the device.  Named windows are
+computed once per (spec, length, periodic) and shared read-only.
 """
 
 from __future__ import annotations
 
-import functools
+import collections
 
 import numpy as np
 import torch
-from scipy.signal import get_window as _sp_get_window
+from scipy.signal import get_window as _scipy_window
 
 __all__ = ["get_window"]
 
+_MAX_CACHED = 256
+_cache: "collections.OrderedDict" = collections.OrderedDict()
 
-@functools.lru_cache(maxsize=256)
-def _window_np(window, Nx: int, fftbins: bool = True) -> np.ndarray:
-    """Cached float64 window values (hashable window spec only)."""
-    w = _sp_get_window(window, Nx, fftbins=fftbins)
-    w = np.ascontiguousarray(w, dtype=np.float64)
-    w.setflags(write=False)
-    return w
+
+def window_values(spec, length: int, periodic: bool = True) -> np.ndarray:
+    """float64 values of a named window spec (a name, a (name, params...) tuple
+    or a Kaiser beta), read-only and shared between callers."""
+    key = (spec, int(length), bool(periodic))
+    values = _cache.get(key)
+    if values is None:
+        values = np.array(_scipy_window(spec, int(length), fftbins=bool(periodic)),
+                          dtype=np.float64, copy=True)
+        values.setflags(write=False)
+        _cache[key] = values
+        if len(_cache) > _MAX_CACHED:
+            _cache.popitem(last=False)
+    else:
+        _cache.move_to_end(key)
+    return values
 
 
 def get_window(window, Nx: int, fftbins: bool = True, device=None) -> torch.Tensor:
-    """Return a window of a given length and type as a float32 tensor.
+    """The window ``window`` of length ``Nx`` as a float32 tensor.
 
-    Accepts everything ``scipy.signal.get_window`` accepts: a name
-    (``"hann"``), a ``(name, param)`` tuple (``("kaiser", 8.6)``) or a float
-    (kaiser beta).
-    """
+    ``window`` is anything scipy.signal.get_window takes: a name ("hann"), a
+    tuple of a name and its parameters (("kaiser", 8.6)) or a float (a Kaiser
+    beta).  ``fftbins`` selects the periodic form (True) or the symmetric
+    one.  The tensor is on ``device`` (default: the result device)."""
     from ._device import result_device
 
-    key = tuple(window) if isinstance(window, (list, tuple)) else window
-    w = _window_np(key, int(Nx), bool(fftbins))
-    return torch.from_numpy(w.astype(np.float32)).to(device or result_device())
+    spec = tuple(window) if isinstance(window, list) else window
+    values = window_values(spec, Nx, fftbins)
+    return torch.tensor(values, dtype=torch.float32, device=device or result_device())

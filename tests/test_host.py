@@ -11,7 +11,7 @@ import torch
 
 import mlx_signal_amd as msig
 from _utils import assert_close, make_sig
-from mlx_signal_amd import _array, _device, spectral
+from mlx_signal_amd import _array, _device, _plan, spectral
 
 
 def test_set_config_is_atomic():
@@ -133,41 +133,57 @@ def test_argument_errors():
     with pytest.raises(ValueError, match="mode"):
         msig.spectrogram(x, mode="power")
     with pytest.raises(ValueError, match="mode"):
-        spectral._spectral_helper(x, x, mode="bad")
+        spectral._estimate(x, x, mode="bad")
     with pytest.raises(ValueError, match="boundary"):
-        spectral._spectral_helper(x, x, boundary="wrap")
+        spectral._estimate(x, x, boundary="wrap")
     with pytest.raises(ValueError, match="equal"):
-        spectral._spectral_helper(x, np.zeros(4096, np.float32), mode="stft")
+        spectral._estimate(x, np.zeros(4096, np.float32), mode="stft")
 
 
 def test_triage_segments_matches_reference_rules():
-    w, n = spectral._triage_segments("hann", None, 1000)
+    w, n = _plan.resolve_window("hann", None, 1000)
     assert n == 256 and w.shape == (256,)
     with pytest.warns(UserWarning, match="nperseg"):
-        w, n = spectral._triage_segments("hann", 512, 100)
+        w, n = _plan.resolve_window("hann", 512, 100)
     assert n == 100
     with pytest.raises(ValueError, match="1-D"):
-        spectral._triage_segments(np.ones((2, 2)), None, 100)
+        _plan.resolve_window(np.ones((2, 2)), None, 100)
     with pytest.raises(ValueError, match="longer"):
-        spectral._triage_segments(np.ones(200), None, 100)
+        _plan.resolve_window(np.ones(200), None, 100)
     with pytest.raises(ValueError, match="different"):
-        spectral._triage_segments(np.ones(64), 32, 100)
-    np.testing.assert_array_equal(spectral._onesided_factor_np(8), [1, 2, 2, 2, 1])
-    np.testing.assert_array_equal(spectral._onesided_factor_np(7), [1, 2, 2, 2])
-    assert abs(spectral._median_bias(3) - 5 / 6) < 1e-15
+        _plan.resolve_window(np.ones(64), 32, 100)
+    with pytest.raises(ValueError, match="positive"):
+        _plan.check_nperseg(0)
+    with pytest.raises(ValueError, match="nfft"):
+        _plan.resolve_lengths(64, None, 32)
+    with pytest.raises(ValueError, match="noverlap"):
+        _plan.resolve_lengths(64, 64, None)
+    assert _plan.resolve_lengths(64, None, None) == (32, 32, 64)
+    np.testing.assert_array_equal(_plan.onesided_factor(8), [1, 2, 2, 2, 1])
+    np.testing.assert_array_equal(_plan.onesided_factor(7), [1, 2, 2, 2])
+    assert abs(_plan.median_bias(3) - 5 / 6) < 1e-15
+    from scipy.signal._spectral_py import _median_bias as scipy_bias
+
+    for n in (1, 2, 3, 4, 9, 10, 101, 4096):
+        assert abs(_plan.median_bias(n) - scipy_bias(n)) < 1e-14, n
+    for n, nps, step in ((1000, 256, 128), (1000, 100, 37), (256, 256, 1), (999, 64, 64)):
+        nseg = _plan.segment_count(n, nps, step)
+        assert nseg == len(range(0, n - nps + 1, step))
+        pad = _plan.padding_for(n, nps, step)
+        assert (n + pad - nps) % step == 0 and 0 <= pad < nps
 
 
 def test_median_last_matches_numpy(rng):
     a = rng.standard_normal((3, 5, 8)).astype(np.float32)
     for n in (7, 8):
         t = torch.from_numpy(a[..., :n])
-        np.testing.assert_allclose(spectral._median_last(t).numpy(), np.median(a[..., :n], axis=-1))
+        np.testing.assert_allclose(spectral._median_over_last(t).numpy(), np.median(a[..., :n], axis=-1))
 
 
 def test_unwrap_matches_numpy(rng):
     p = np.cumsum(rng.uniform(-3, 3, (4, 50)), axis=1).astype(np.float32)
     for ax in (0, 1):
-        np.testing.assert_allclose(spectral._unwrap(torch.from_numpy(p), dim=ax).numpy(),
+        np.testing.assert_allclose(spectral._unwrap_phase(torch.from_numpy(p), dim=ax).numpy(),
                                    np.unwrap(p, axis=ax), rtol=1e-5, atol=1e-4)
 
 
@@ -192,11 +208,11 @@ def test_composed_detrend_matches_oracle(rng):
     d = make_sig(rng, (3, 7, 64))
     w = np.hanning(64).astype(np.float32)
     for kind in (False, "constant", "linear"):
-        got = spectral._detrend_and_window(torch.from_numpy(d), kind, torch.from_numpy(w)).numpy()
+        got = spectral._window_segments(torch.from_numpy(d), kind, torch.from_numpy(w)).numpy()
         ref = osp._detrend(d.astype(np.float64), kind) * w
         np.testing.assert_allclose(got, ref, rtol=1e-4, atol=1e-5)
     with pytest.raises(ValueError):
-        spectral._detrend_and_window(torch.from_numpy(d), "quadratic", torch.from_numpy(w))
+        spectral._window_segments(torch.from_numpy(d), "quadratic", torch.from_numpy(w))
 
 
 def test_istft_nola_helper_matches_oracle_rule():
@@ -241,15 +257,15 @@ def test_window_sums_cached_only_for_read_only_windows():
     """The planner keeps (sum w, sum w^2) per named-window array (read-only,
     from the window cache) and recomputes them for a caller's own array,
     which may change between calls."""
-    from mlx_signal_amd.windows import _window_np
+    from mlx_signal_amd.windows import window_values
 
-    w = _window_np("hann", 256)
+    w = window_values("hann", 256)
     assert not w.flags.writeable
-    s1 = spectral._win_sums(w)
+    s1 = _plan.window_sums(w)
     assert s1 == (float(w.sum()), float((w * w).sum()))
-    assert spectral._win_sums(w) is s1  # cached for the same read-only array
+    assert _plan.window_sums(w) is s1  # cached for the same read-only array
     mine = np.hanning(256)
-    a = spectral._win_sums(mine)
+    a = _plan.window_sums(mine)
     mine *= 2.0
-    b = spectral._win_sums(mine)
+    b = _plan.window_sums(mine)
     assert b[0] == pytest.approx(2 * a[0]) and b[1] == pytest.approx(4 * a[1])
