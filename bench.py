@@ -68,6 +68,14 @@ def make_inputs(torch, rows, n, seed, dev, with_y=True):
     return x.contiguous(), y.contiguous()
 
 
+def max_err_rows(got, ref):
+    """Largest per-bin error of each row relative to that row's FP64 peak
+    (the parity metric), maxed over the rows checked."""
+    got = np.asarray(got).reshape(len(ref), -1).astype(np.complex128)
+    ref = np.asarray(ref).reshape(len(ref), -1).astype(np.complex128)
+    return float(max(np.max(np.abs(g - r)) / np.max(np.abs(r)) for g, r in zip(got, ref)))
+
+
 WARM_MS = 80.0  # back-to-back work before any timed region
 
 
@@ -120,9 +128,15 @@ def time_steps(torch, fn, bufs, steps, warmup, kernel_timing=False):
     return wall / steps * 1e3, statistics.median(ev_ms), ev_ms, kernel_ms
 
 
+TRAFFIC_PROFILE = "profiles/welch_64x2p20_traffic.json"
+
+
 def traffic_from_profile():
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
-    p = ROOT / "profiles" / "welch_64x2p20_traffic.json"
+    """HBM bytes per launch of the dominant kernel, read from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated on copies of
+    known size), or None.  It is a committed measurement of the same kernel
+    and workload, not a counter read in this run."""
+    p = ROOT / TRAFFIC_PROFILE
     if p.exists():
         try:
             return json.loads(p.read_text()).get("hbm_bytes_per_launch")
@@ -173,7 +187,9 @@ def _cpu_welch_rows(bounds):
 
 def cpu_all_cores():
     """The same scipy route on the headline workload sharded by channel over
-    the host cores (one forked process per core, up to 16), median of 3."""
+    the host cores (one forked process per core, up to 16), median of 3.
+    Runs before this process initialises the GPU, so the forked workers
+    inherit no HIP state; the pool is shut down (workers joined) on exit."""
     import concurrent.futures
     import multiprocessing
 
@@ -186,13 +202,16 @@ def cpu_all_cores():
     edges = np.linspace(0, rows, cores + 1).astype(int)
     chunks = [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:]) if b > a]
     ctx = multiprocessing.get_context("fork")
-    with concurrent.futures.ProcessPoolExecutor(len(chunks), mp_context=ctx) as ex:
+    ex = concurrent.futures.ProcessPoolExecutor(len(chunks), mp_context=ctx)
+    try:
         list(ex.map(_cpu_welch_rows, chunks))
         ts = []
         for _ in range(3):
             t0 = time.perf_counter()
             list(ex.map(_cpu_welch_rows, chunks))
             ts.append(time.perf_counter() - t0)
+    finally:
+        ex.shutdown(wait=True, cancel_futures=True)
     _CPU_X = None
     t = statistics.median(ts)
     return {"value": rows * NS / t, "unit": "samples/s", "cores": cores,
@@ -441,8 +460,24 @@ def bench_serving(torch, msig, dev):
     out = {"requests": len(sched), "offered_req_s": 2000.0, "samples": nsamp,
            "input_bytes": nbytes, "offered_GBps": nbytes / sched[-1][0] / 1e9}
     srv = SpectralServer(hbm_budget=96 << 30, streams=3, max_inflight=3, tuning=tuning)
-    out["batched"] = latency_stats(srv.serve(sched))
+    recs = srv.serve(sched)
+    out["batched"] = latency_stats(recs)
     out["batched"]["batches"] = srv.batches
+    # accuracy: the 6 smallest served results against the FP64 oracle
+    import oracle
+
+    torch.cuda.synchronize()
+    small = sorted(range(len(sched)), key=lambda i: sched[i][1].x.numel())[:6]
+    err = 0.0
+    for i in small:
+        r = sched[i][1]
+        kw = dict(fs=r.fs, nperseg=r.nperseg)
+        xh = r.x.cpu().numpy()
+        ref = (oracle.welch(xh, **kw)[1] if r.kind == "welch"
+               else oracle.csd(xh, r.y.cpu().numpy(), **kw)[1])
+        err = max(err, max_err_rows(recs[i].result.cpu().numpy(), ref))
+    out["batched"]["max_err_vs_fp64_peak_sampled"] = err
+    del recs
     torch.cuda.synchronize()
     out["plain"] = _stats(_plain_serve(torch, msig, sched))
     torch.cuda.synchronize()
@@ -482,6 +517,14 @@ def main():
               "would shard by channel row with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
 
+    # CPU legs first, before this process touches the GPU: the all-core leg
+    # forks workers, which must not inherit HIP runtime state
+    cpu = cpu_all = None
+    if not args.no_cpu and not args.precision_only:
+        cpu = cpu_baseline()
+        if not args.no_secondary:
+            cpu_all = cpu_all_cores()
+
     import torch
 
     import mlx_signal_amd as msig
@@ -511,11 +554,12 @@ def main():
     # kernel; the finalize kernel reads only the block partials)
     achieved = B / (ms_kernel * 1e-3) / 1e9
 
-    # accuracy gate on the measured data: 2 channels vs the FP64 oracle
+    # accuracy gate on the measured data: 3 channels vs the FP64 oracle
     _, p = step(xs[0])
-    xr = xs[0][:2].cpu().numpy()
+    hrows = [0, C // 2 - 1, C - 1]
+    xr = xs[0][hrows].cpu().numpy()
     ref = oracle.welch(xr, fs=FS, window="hann", nperseg=NPERSEG, noverlap=NOVERLAP)[1]
-    err = float(np.max(np.abs(p[:2].cpu().numpy() - ref)) / np.max(np.abs(ref)))
+    err = max_err_rows(p[hrows].cpu().numpy(), ref)
 
     if args.copy_ref:
         src = xs[0]
@@ -531,26 +575,40 @@ def main():
         x3, y3 = make_inputs(torch, C, NS, 12, dev)
         pairs = [(x2, y2), (x3, y3)]
         kw = dict(fs=FS, window="hann", nperseg=NPERSEG, noverlap=NOVERLAP)
-        for name, fn, nbytes in [
-            ("csd", lambda b: msig.csd(b[0], b[1], **kw), 8 * C * NS + 8 * C * bins),
-            ("coherence", lambda b: msig.coherence(b[0], b[1], **kw), 8 * C * NS + 4 * C * bins),
+        rows = [0, C // 2 - 1, C - 1]
+        xh, yh = x2[rows].cpu().numpy(), y2[rows].cpu().numpy()
+        for name, fn, nbytes, ref in [
+            ("csd", lambda b: msig.csd(b[0], b[1], **kw), 8 * C * NS + 8 * C * bins,
+             lambda: oracle.csd(xh, yh, **kw)[1]),
+            ("coherence", lambda b: msig.coherence(b[0], b[1], **kw), 8 * C * NS + 4 * C * bins,
+             lambda: oracle.coherence(xh, yh, **kw)[1]),
         ]:
             w, e, _, _ = time_steps(torch, fn, pairs, max(5, args.steps // 2), 2)
+            got = fn(pairs[0])[1][rows].cpu().numpy()
             secondary[name] = {"ms_event": e, "ms_wall": w, "GBps": nbytes / (e * 1e-3) / 1e9,
-                               "pairs_per_s": C * NS / (e * 1e-3)}
+                               "pairs_per_s": C * NS / (e * 1e-3),
+                               "max_err_vs_fp64_peak": max_err_rows(got, ref()),
+                               "err_rows": rows}
         del x2, y2, x3, y3, pairs
         xs16 = [x[:16] for x in xs]
         nseg_stft = (NS + 1024 - 1024) // 512 + 1
         nseg_sp = (NS - 1024) // 512 + 1
-        for name, fn, nbytes in [
+        srows = [0, 15]
+        xs16h = xs16[0][srows].cpu().numpy()
+        for name, fn, nbytes, ref in [
             ("stft_16x2^20", lambda b: msig.stft(b, nperseg=1024),
-             4 * 16 * NS + 8 * 16 * nseg_stft * bins),
+             4 * 16 * NS + 8 * 16 * nseg_stft * bins,
+             lambda: oracle.stft(xs16h, nperseg=1024)[2]),
             ("spectrogram_16x2^20", lambda b: msig.spectrogram(b, nperseg=1024, noverlap=512),
-             4 * 16 * NS + 4 * 16 * nseg_sp * bins),
+             4 * 16 * NS + 4 * 16 * nseg_sp * bins,
+             lambda: oracle.spectrogram(xs16h, nperseg=1024, noverlap=512)[2]),
         ]:
             w, e, _, _ = time_steps(torch, fn, xs16, max(5, args.steps // 2), 2)
+            got = fn(xs16[0])[2][srows].cpu().numpy()
             secondary[name] = {"ms_event": e, "ms_wall": w, "GBps": nbytes / (e * 1e-3) / 1e9,
-                               "samples_per_s": 16 * NS / (e * 1e-3)}
+                               "samples_per_s": 16 * NS / (e * 1e-3),
+                               "max_err_vs_fp64_peak": max_err_rows(got, ref()),
+                               "err_rows": srows}
         # istft of the 16 x 2^20 stft (hop = N/2: inverse and overlap-add in one kernel)
         zs = [msig.stft(b, nperseg=1024)[2] for b in xs16]
         nseg_i = zs[0].shape[-1]
@@ -613,13 +671,16 @@ def main():
         "step_GBps": B / (ms_ev * 1e-3) / 1e9,
         "max_err_vs_fp64_peak": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile()},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
+                     "traffic_source": f"{TRAFFIC_PROFILE}: committed rocprofv3 --pmc "
+                                       "FETCH_SIZE/WRITE_SIZE passes of this kernel and "
+                                       "workload (not measured in this run)"},
         "secondary": secondary,
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
-        if not args.no_secondary:
-            secondary["cpu_all_cores"] = cpu_all_cores()
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    if cpu_all is not None:
+        secondary["cpu_all_cores"] = cpu_all
     print(json.dumps(out), flush=True)
 
 

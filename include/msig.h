@@ -311,7 +311,7 @@ int msig_istft_ola(const void* z, int64_t rows, int32_t nseg, int32_t nperseg,
  * (channels 1 = real, 2 = interleaved complex); for i = first + j, j < count:
  *   out[row][j][c] = sum_s frames[row][s][i - s*hop][c] / norm(i),
  *   norm(i) = sum_s win[i - s*hop]^2 (the divide only where norm > 1e-10),
- * summed over the segments covering i in increasing s.  rows <= 65535;
+ * summed over the segments covering i in increasing s (any row count);
  * first + count <= nperseg + (nseg - 1) * hop. */
 int msig_overlap_add(const float* frames, int64_t rows, int32_t nseg, int32_t nperseg,
                      int32_t hop, int32_t channels, const float* win, int64_t first,

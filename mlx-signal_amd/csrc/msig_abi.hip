@@ -320,31 +320,36 @@ struct OlaArgs {
   const float* frames;
   const float* win;
   float* out;
-  int64_t first, count;
+  int64_t first, count, rows;
   int32_t nseg, nperseg, hop, channels;
 };
 
+// One thread per output value, grid-strided over all rows (64-bit indices:
+// any row count).
 __global__ __launch_bounds__(256) void k_overlap_add(OlaArgs o) {
-  const int64_t row = blockIdx.y;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // j * channels + c
-  const int64_t total = o.count * o.channels;
-  if (idx >= total) return;
-  const int c = (int)(idx % o.channels);
-  const int64_t i = o.first + idx / o.channels;
-  int64_t s_hi = i / o.hop;
-  if (s_hi > o.nseg - 1) s_hi = o.nseg - 1;
-  const int64_t s_lo = i >= o.nperseg ? (i - o.nperseg) / o.hop + 1 : 0;
-  const MSIG_GLOBAL float* fr =
-      as_global(o.frames) + row * (int64_t)o.nseg * o.nperseg * o.channels;
+  const int64_t per_row = o.count * o.channels;
+  const int64_t total = o.rows * per_row;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
   const MSIG_GLOBAL float* w = as_global(o.win);
-  float acc = 0.f, nrm = 0.f;
-  for (int64_t s = s_lo; s <= s_hi; ++s) {
-    const int64_t off = i - s * o.hop;
-    acc += fr[(s * o.nperseg + off) * o.channels + c];
-    const float wv = w[off];
-    nrm = fmaf(wv, wv, nrm);
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+    const int64_t row = g / per_row;
+    const int64_t idx = g - row * per_row;  // j * channels + c
+    const int c = (int)(idx % o.channels);
+    const int64_t i = o.first + idx / o.channels;
+    int64_t s_hi = i / o.hop;
+    if (s_hi > o.nseg - 1) s_hi = o.nseg - 1;
+    const int64_t s_lo = i >= o.nperseg ? (i - o.nperseg) / o.hop + 1 : 0;
+    const MSIG_GLOBAL float* fr =
+        as_global(o.frames) + row * (int64_t)o.nseg * o.nperseg * o.channels;
+    float acc = 0.f, nrm = 0.f;
+    for (int64_t s = s_lo; s <= s_hi; ++s) {
+      const int64_t off = i - s * o.hop;
+      acc += fr[(s * o.nperseg + off) * o.channels + c];
+      const float wv = w[off];
+      nrm = fmaf(wv, wv, nrm);
+    }
+    as_global_mut(o.out)[g] = acc / (nrm > 1e-10f ? nrm : 1.f);
   }
-  as_global_mut(o.out)[row * total + idx] = acc / (nrm > 1e-10f ? nrm : 1.f);
 }
 
 // ---------------------------------------------------------------- median
@@ -1511,11 +1516,12 @@ int msig_overlap_add(const float* frames, int64_t rows, int32_t nseg, int32_t np
   if (first < 0 || count < 0 || first + count > outlen)
     return fail(MSIG_EINVAL, "samples [%lld, %lld) outside the %lld-sample output",
                 (long long)first, (long long)(first + count), (long long)outlen);
-  if (rows > 65535) return fail(MSIG_EINVAL, "rows > 65535");
   if (count == 0) return 0;
-  OlaArgs o{frames, win, out, first, count, nseg, nperseg, hop, channels};
-  const int64_t total = count * channels;
-  const dim3 grid((unsigned)((total + 255) / 256), (unsigned)rows), block(256);
+  OlaArgs o{frames, win, out, first, count, rows, nseg, nperseg, hop, channels};
+  const int64_t total = rows * count * channels;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > ((int64_t)1 << 20)) blocks = (int64_t)1 << 20;  // the rest grid-strided
+  const dim3 grid((unsigned)blocks), block(256);
   hipLaunchKernelGGL(k_overlap_add, grid, block, 0, static_cast<hipStream_t>(stream), o);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "overlap-add kernel launch");

@@ -3,10 +3,11 @@
 No counterpart in the reference (a library of synchronous calls).  Requests
 (welch / csd / coherence, see :class:`SpectralRequest`) arrive over time;
 the loop admits them while an HBM budget and a bounded number of in-flight
-batches allow, turns everything admitted since the last launch into one
-ragged batch (:class:`SpectralBatch`: one persistent launch per FFT size plus
-one finalize), and launches it on the next of a few HIP streams, so batches
-formed while others are still running overlap on the device.  Completion
+batches allow, turns queued requests into ragged batches
+(:class:`SpectralBatch`) bounded by estimated GPU time, and launches them on
+a few HIP streams: small requests on express streams, large ones on a bulk
+stream, so batches formed while others are still running overlap on the
+device and a small request does not wait behind a large one.  Completion
 is observed with stream events; finished batches return their memory to the
 budget and their requests' results to the caller.
 
@@ -45,6 +46,7 @@ class ServedRequest:
     result: torch.Tensor | None = None
     freqs: torch.Tensor | None = None
     nbytes: int = 0
+    io_bytes: int = 0
 
     @property
     def latency(self) -> float:
@@ -58,21 +60,34 @@ class _InFlight:
         self.event = event
         self.stream = stream
         self.nbytes = nbytes
+        self.express = True
 
 
 class SpectralServer:
-    """Admit by HBM budget, batch continuously, overlap batches on streams."""
+    """Admit by HBM budget, batch continuously, overlap batches on streams.
+
+    Two lanes share the GPU.  Requests whose estimated GPU time (bytes
+    moved / ``rate_gbps``) is at most ``express_ms`` go to the express lane:
+    batches of them, formed in arrival order and bounded by ``batch_ms`` of
+    estimated time and ``max_batch`` requests, run on their own streams, so a
+    small request never waits behind a large one's kernels.  Larger requests
+    go to the bulk lane in arrival order, alone or grouped up to ``batch_ms``,
+    on one stream.  ``max_inflight`` bounds the batches in flight per lane."""
 
     def __init__(self, hbm_budget: int = 64 << 30, streams: int = 3, max_inflight: int = 3,
-                 max_batch: int = 256, tuning=None, device=None):
+                 max_batch: int = 256, tuning=None, device=None, batch_ms: float = 1.0,
+                 express_ms: float = 0.25, rate_gbps: float = 2000.0):
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.budget = int(hbm_budget)
-        # default-priority streams only
-        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(streams)]
+        # default-priority streams only: the first carries the bulk lane
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(max(2, streams))]
         self.max_inflight = max_inflight
         self.max_batch = max_batch
         self.tuning = tuning
-        self.queue: collections.deque = collections.deque()
+        self.batch_ms = batch_ms
+        self.express_ms = express_ms
+        self.rate = rate_gbps * 1e6  # bytes per ms
+        self._lanes = {True: collections.deque(), False: collections.deque()}  # express, bulk
         self.inflight: list = []
         self.bytes_in_flight = 0
         self.completed: list = []
@@ -83,28 +98,57 @@ class SpectralServer:
     def now(self) -> float:
         return time.perf_counter() - self.t0
 
+    @property
+    def queue(self) -> list:
+        """Requests waiting for admission (express lane first)."""
+        return list(self._lanes[True]) + list(self._lanes[False])
+
+    def est_ms(self, s) -> float:
+        """Estimated GPU time of a request: its input and output bytes at the
+        kernels' rate."""
+        return s.io_bytes / self.rate
+
     def submit(self, request: SpectralRequest, arrival: float | None = None) -> ServedRequest:
         nb = request_bytes(request)
         if nb > self.budget:
             raise ValueError(f"request needs {nb} bytes, more than the HBM budget {self.budget}")
         s = ServedRequest(request, self.now() if arrival is None else arrival, nbytes=nb)
-        self.queue.append(s)
+        s.io_bytes = 4 * request.x.numel() * (2 if request.y is not None else 1)
+        self._lanes[self.est_ms(s) <= self.express_ms].append(s)
         return s
 
-    def _launch(self):
-        if not self.queue or len(self.inflight) >= self.max_inflight:
-            return False
-        take, nb = [], 0
-        while self.queue and len(take) < self.max_batch:
-            s = self.queue[0]
-            if self.bytes_in_flight + nb + s.nbytes > self.budget:
+    def _lane_inflight(self, express: bool) -> int:
+        return sum(1 for f in self.inflight if f.express == express)
+
+    def _take(self, express: bool):
+        """The next batch of one lane from the queue (arrival order within
+        the lane), within the budget, batch_ms and max_batch."""
+        take, nb, ms = [], 0, 0.0
+        for s in self._lanes[express]:
+            t = self.est_ms(s)
+            if take and (ms + t > self.batch_ms or len(take) >= self.max_batch):
                 break
-            take.append(self.queue.popleft())
+            if self.bytes_in_flight + nb + s.nbytes > self.budget:
+                break  # keep the lane's order: wait for memory
+            take.append(s)
             nb += s.nbytes
+            ms += t
+        return take, nb
+
+    def _launch(self, express: bool) -> bool:
+        if self._lane_inflight(express) >= self.max_inflight:
+            return False
+        take, nb = self._take(express)
         if not take:
             return False
-        stream = self.streams[self._next_stream % len(self.streams)]
-        self._next_stream += 1
+        lane = self._lanes[express]
+        for _ in take:
+            lane.popleft()
+        if express:
+            stream = self.streams[1 + self._next_stream % (len(self.streams) - 1)]
+            self._next_stream += 1
+        else:
+            stream = self.streams[0]
         t = self.now()
         for s in take:
             s.admitted = t
@@ -115,7 +159,9 @@ class SpectralServer:
             b.run()
             ev = torch.cuda.Event()
             ev.record(stream)
-        self.inflight.append(_InFlight(b, take, ev, stream, nb))
+        f = _InFlight(b, take, ev, stream, nb)
+        f.express = express
+        self.inflight.append(f)
         self.bytes_in_flight += nb
         self.batches += 1
         return True
@@ -141,13 +187,19 @@ class SpectralServer:
         self.inflight = still
 
     def step(self) -> None:
-        """Reap finished batches, then launch what the budget admits."""
+        """Reap finished batches, then launch what the budget admits, express
+        lane first."""
         self._reap()
-        while self._launch():
+        while self._launch(True):
+            pass
+        while self._launch(False):
             pass
 
+    def _waiting(self) -> bool:
+        return bool(self._lanes[True] or self._lanes[False])
+
     def drain(self) -> None:
-        while self.queue or self.inflight:
+        while self._waiting() or self.inflight:
             self.step()
 
     def serve(self, arrivals):
@@ -156,7 +208,7 @@ class SpectralServer:
         self.t0 = time.perf_counter()
         pending = collections.deque(sorted(arrivals, key=lambda a: a[0]))
         records = []
-        while pending or self.queue or self.inflight:
+        while pending or self._waiting() or self.inflight:
             now = self.now()
             while pending and pending[0][0] <= now:
                 t, r = pending.popleft()

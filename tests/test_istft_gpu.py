@@ -13,7 +13,7 @@ import torch
 
 import mlx_signal_amd as msig
 import oracle
-from _utils import peak_rel_err
+from _utils import assert_peak_close, peak_rel_err
 from mlx_signal_amd import _kernels
 
 pytestmark = pytest.mark.gpu
@@ -243,3 +243,15 @@ def test_istft_errors(rng):
         msig.istft(z, scaling="bogus")
     with pytest.raises(ValueError):
         msig.istft(z, window=np.ones(10), nperseg=64)
+
+
+def test_overlap_add_more_than_65535_rows(rng):
+    """The overlap-add kernel strides over all rows (no 65535 grid limit):
+    istft with a hop other than nperseg/2 on 70000 short rows."""
+    rows, nps, hop, nseg = 70_000, 16, 4, 3
+    z = (rng.standard_normal((rows, nps // 2 + 1, nseg))
+         + 1j * rng.standard_normal((rows, nps // 2 + 1, nseg))).astype(np.complex64)
+    t, x = msig.istft(torch.from_numpy(z).cuda(), nperseg=nps, noverlap=nps - hop)
+    for r in (0, 65_535, 69_999):
+        _, ref = oracle.istft(z[r], nperseg=nps, noverlap=nps - hop)
+        assert_peak_close(x[r], ref, 1e-5)
