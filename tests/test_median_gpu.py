@@ -77,3 +77,26 @@ def test_median_axis_and_two_sided(rng):
     _, p2 = msig.welch(x[:, 0], nperseg=128, average="median", return_onesided=False)
     assert_peak_close(p2, oracle.welch(x[:, 0], nperseg=128, average="median",
                                        return_onesided=False)[1], 1e-5)
+
+
+@pytest.mark.parametrize("nseg", [2047, 2048, 4095, 4096])
+def test_median_kernel_adversarial_columns(rng, nseg):
+    """Columns that stress the radix select: exponential power values, one
+    huge outlier over the rest (the range-relative digits then squeeze the
+    other keys into one bucket), an outlier over a narrow cluster, a column
+    split into two far groups (for even counts the two middle values lie in
+    different buckets), ties and a constant column.  All must equal
+    numpy.median exactly."""
+    rows, bins = 2, 12
+    a = rng.exponential(size=(rows, nseg, bins)).astype(np.float32)
+    a[:, 0, 1] = 1e30                                      # outlier: fallback
+    a[:, :, 2] = 1.0 + rng.random((rows, nseg)).astype(np.float32) * 1e-3
+    a[:, 0, 2] = 1e30                                      # outlier over a narrow cluster
+    half = nseg // 2
+    lo = 10.0 ** rng.uniform(-10, 0, (rows, half))
+    hi = 10.0 ** rng.uniform(10, 20, (rows, nseg - half))
+    a[:, :, 3] = np.concatenate([lo, hi], axis=1).astype(np.float32)  # split column
+    a[:, :, 4] = np.round(rng.exponential(size=(rows, nseg)) * 4)    # ties
+    a[:, :, 5] = 7.0                                                 # constant
+    got = _kernels.median_frames(torch.from_numpy(a).cuda(), 1.0, False, 0).cpu().numpy()
+    np.testing.assert_array_equal(got, np.median(a, axis=1).astype(np.float32))
