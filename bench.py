@@ -312,7 +312,8 @@ def ragged_requests(torch, dev, n_inst=512, seed=7):
 
 def bench_ragged(torch, msig, dev, reps=3):
     """Ragged batch of 512 instances: one plain call per request vs one ragged batch
-    (autotuned, and replayed from a HIP graph)."""
+    (autotuned, and replayed from a HIP graph), and the same batch with one
+    launch per (nfft, kind) class instead of the single persistent launch."""
     import oracle
     from mlx_signal_amd.batch import SpectralBatch
     from mlx_signal_amd.tuning import cached_tuning
@@ -347,22 +348,35 @@ def bench_ragged(torch, msig, dev, reps=3):
     ms_batch = timed(b.run)
     g = b.graph()
     ms_graph = timed(g.replay)
-    # accuracy: 6 of the smallest requests against the FP64 oracle
+    # the same requests with one persistent launch per (nfft, kind) class
+    tuning_pc = cached_tuning((256, 1024, 4096), per_class_launches=True)
+    bp = SpectralBatch(reqs, tuning=tuning_pc, per_class_launches=True)
+    gp = bp.graph()
+    ms_graph_per_class = timed(gp.replay)
+    launches_per_class = bp.launches
+    # accuracy of both plans: 6 of the smallest requests against the FP64 oracle
     small = sorted(range(len(reqs)), key=lambda i: reqs[i].x.numel())[:6]
-    err = 0.0
+    err = err_pc = 0.0
     for i in small:
         r = reqs[i]
         kw = dict(fs=r.fs, window=r.window, nperseg=r.nperseg, noverlap=r.noverlap)
         xh = r.x.cpu().numpy()
         ref = (oracle.welch(xh, **kw)[1] if r.kind == "welch"
                else oracle.csd(xh, r.y.cpu().numpy(), **kw)[1])
-        got = b.outputs[i].cpu().numpy()
-        err = max(err, float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))))
+        peak = np.max(np.abs(ref))
+        err = max(err, float(np.max(np.abs(b.outputs[i].cpu().numpy() - ref)) / peak))
+        err_pc = max(err_pc, float(np.max(np.abs(bp.outputs[i].cpu().numpy() - ref)) / peak))
+    bp.close()
+    del gp, bp
     out = {
         "instances": len(reqs), "samples": b.samples, "algorithmic_bytes": b.bytes,
         "launches_per_batch": b.launches, "tuning": {str(k): v for k, v in tuning.items()},
         "ms_one_call_per_request": ms_plain, "ms_batch": ms_batch, "ms_graph": ms_graph,
         "speedup_batch": ms_plain / ms_batch, "speedup_graph": ms_plain / ms_graph,
+        "ms_graph_per_class_launches": ms_graph_per_class,
+        "tuning_per_class": {str(k): v for k, v in tuning_pc.items()},
+        "launches_per_class_batch": launches_per_class,
+        "max_err_vs_fp64_peak_sampled_per_class": err_pc,
         "samples_per_s_graph": b.samples / (ms_graph * 1e-3),
         "GBps_graph": b.bytes / (ms_graph * 1e-3) / 1e9,
         "max_err_vs_fp64_peak_sampled": err,

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define MSIG_ABI_VERSION 7
+#define MSIG_ABI_VERSION 8
 
 #define MSIG_EINVAL (-22)
 #define MSIG_ENOTSUP (-95)
@@ -200,12 +200,16 @@ int msig_coherence_mean(const float* x, const float* y, int64_t rows, int64_t n,
 
 /* ---- ragged batches (many mean jobs of any shape in one submission) ----
  * No counterpart in the reference (it launches one kernel per call).  A plan
- * partitions the jobs' work by FFT size, uploads per-job descriptors and work
- * queues to the device once, and each run is kernels only (one persistent
- * launch per size class plus one finalize launch), so a run can be captured
- * in a hipGraph and replayed.  Every job is computed with the same block
- * geometry as a standalone msig_spectral_mean call with the same tuning, so
- * its result is bitwise the same alone and in any batch. */
+ * uploads per-job descriptors and work queues to the device once, and each
+ * run is kernels only, so a run can be captured in a hipGraph and replayed:
+ * one persistent launch walks the work of every welch / csd job on real
+ * input with hop = nfft/2 (all FFT sizes in one kernel), one persistent
+ * launch per (nfft, kind, input) class covers any other job (coherence,
+ * other hops, complex input, radix schedule 1), and one launch finalizes
+ * every job.  Every job is computed with the same block geometry as a
+ * standalone msig_spectral_mean call with the same tuning, so its result is
+ * bitwise the same alone and in any batch.  Runs of one plan on one device
+ * must not overlap (they share the workspace and the work counter). */
 
 typedef struct msig_batch_job {
   msig_job job;     /* a mean kind: MSIG_PSD_MEAN, MSIG_CSD_MEAN, MSIG_COH_TRIO_MEAN, MSIG_COHERENCE */
@@ -219,8 +223,10 @@ typedef struct msig_batch_job {
 
 /* Per FFT size (index log2(nfft)): 0 selects the default.
  *   max_run:        most segment pairs one thread group sums sequentially (<= 32)
- *   target_blocks:  blocks a job's geometry aims for (default 1024)
- *   blocks_per_cu:  resident blocks per CU of the persistent launch (default 4)
+ *   target_blocks:  blocks a job's geometry aims for (default: the standalone
+ *                   call's plan, whole waves of resident blocks)
+ *   blocks_per_cu:  resident blocks per CU of a per-class persistent launch
+ *                   (default 4; the single launch runs at its occupancy)
  *   radix_schedule: 0 = radix-16 passes first, remainder radix last (default);
  *                   1 = remainder radix first (nfft = 16^m r, r in {2, 4, 8});
  *                   identical for powers of 16 */
@@ -229,6 +235,9 @@ typedef struct msig_tuning {
   int32_t target_blocks[13];
   int32_t blocks_per_cu[13];
   int32_t radix_schedule[13];
+  int32_t per_class_launches; /* 1: one persistent launch per class for every job
+                                 (blocks_per_cu applies); 0 (default): the
+                                 single launch for welch / csd at hop = nfft/2 */
 } msig_tuning;
 
 typedef struct msig_batch msig_batch; /* opaque plan */
@@ -249,7 +258,8 @@ int msig_batch_workspace_bytes(const msig_batch* plan, size_t* bytes);
  * captured in a hipGraph. */
 int msig_batch_run(const msig_batch* plan, const void* device, void* workspace, size_t ws_bytes,
                    void* stream);
-/* Number of kernel launches one run makes (size classes + 1). */
+/* Number of kernel launches one run makes (2 for a batch of welch / csd jobs at
+ * hop = nfft/2; one more per other class). */
 int msig_batch_launches(const msig_batch* plan);
 int msig_batch_destroy(msig_batch* plan);
 

@@ -852,14 +852,18 @@ struct msig_batch {
   };
   int device = 0;
   std::vector<Class> classes;
+  // the single launch's items occupy queue[0, single_nwork)
+  int32_t single_nwork = 0;
+  int single_grid = 0;
   std::vector<msig::KArgs> jobs;
   std::vector<msig::WorkItem> queue;
   std::vector<msig::FinArgs> fins;
   std::vector<msig::WorkItem> finq;
   int fin_grid = 1;
   size_t ws_bytes = 0;
-  // byte offsets of the four arrays inside the descriptor block
-  size_t off_jobs = 0, off_queue = 0, off_fins = 0, off_finq = 0, desc_bytes = 0;
+  // byte offsets of the arrays and the single launch's 64-bit work counter
+  // inside the descriptor block
+  size_t off_jobs = 0, off_queue = 0, off_fins = 0, off_finq = 0, off_ticket = 0, desc_bytes = 0;
 };
 
 using namespace msig;
@@ -1144,6 +1148,9 @@ int msig_batch_create(const msig_batch_job* jobs, int32_t njobs, const msig_tuni
   std::vector<FinArgs> hf(njobs);
   std::map<std::tuple<int, int, bool, bool>, std::vector<WorkItem>> qs;
   std::vector<WorkItem> finq;
+  // jobs of the single launch, with the estimated cost of one of their blocks
+  std::vector<std::pair<int64_t, int32_t>> single;
+  const bool per_class = tuning && tuning->per_class_launches;
   size_t off = 0;  // floats
   for (int32_t i = 0; i < njobs; ++i) {
     const msig_batch_job& bj = jobs[i];
@@ -1197,14 +1204,55 @@ int msig_batch_create(const msig_batch_job* jobs, int32_t njobs, const msig_tuni
     const bool half = !cplx && j->nperseg == N && 2 * j->nstep == N;
     const int64_t nblk = j->rows * g.bpr;
     if (nblk > INT32_MAX) return fail(MSIG_EINVAL, "job %d: too many blocks", i);
-    auto& q = qs[std::make_tuple(N, kk, cplx, half)];
-    for (int64_t b = 0; b < nblk; ++b) q.push_back(WorkItem{i, (int32_t)b});
+    const int sched = (tuning && tuning->radix_schedule[l2] == 1) ? 1 : 0;
+    if (!per_class && half && kk != K_COH && sched == 0) {
+      hk[i].cls = ragged_class(N, kk);
+      // segment pairs per block x N log2 N, doubled for two signals
+      const int64_t cost = (int64_t)g.gpb * g.run * N * l2 * (kk == K_CSD ? 2 : 1);
+      single.emplace_back(cost, i);
+    } else {
+      auto& q = qs[std::make_tuple(N, kk, cplx, half)];
+      for (int64_t b = 0; b < nblk; ++b) q.push_back(WorkItem{i, (int32_t)b});
+    }
     const int64_t chunks = (f.nbins + FIN_BINS - 1) / FIN_BINS;
     for (int64_t b = 0; b < j->rows * chunks; ++b) finq.push_back(WorkItem{i, (int32_t)b});
     off += ((size_t)j->rows * g.bpr * f.nc * N + 63) / 64 * 64;
   }
   msig_batch* p = new msig_batch();
   p->device = dev;
+  // Single launch: blocks are taken in queue order.  Each class's blocks
+  // are contiguous (a CU then runs one class's code and twiddle table at a
+  // time), classes in decreasing order of their costliest block, and inside
+  // a class the costliest blocks first, jobs of one cost in job order.
+  {
+    std::map<int, int64_t> top;  // class -> its costliest block
+    for (const auto& cj : single) {
+      int64_t& t = top[hk[cj.second].cls];
+      if (cj.first > t) t = cj.first;
+    }
+    std::stable_sort(single.begin(), single.end(),
+                     [&](const std::pair<int64_t, int32_t>& a, const std::pair<int64_t, int32_t>& b) {
+                       const int ca = hk[a.second].cls, cb = hk[b.second].cls;
+                       if (ca != cb) {
+                         const int64_t ta = top[ca], tb = top[cb];
+                         return ta != tb ? ta > tb : ca > cb;
+                       }
+                       return a.first > b.first;
+                     });
+  }
+  for (const auto& cj : single) {
+    const int64_t nblk = hk[cj.second].bpr * jobs[cj.second].job.rows;
+    for (int64_t b = 0; b < nblk; ++b) p->queue.push_back(WorkItem{cj.second, (int32_t)b});
+  }
+  if (p->queue.size() > (size_t)INT32_MAX) {
+    delete p;
+    return fail(MSIG_EINVAL, "too many blocks in one batch");
+  }
+  p->single_nwork = (int32_t)p->queue.size();
+  if (p->single_nwork) {
+    const int64_t cap = (int64_t)ragged_blocks_per_cu() * cus;
+    p->single_grid = (int)(p->single_nwork < cap ? p->single_nwork : cap);
+  }
   p->jobs = std::move(hk);
   p->fins = std::move(hf);
   for (auto& kv : qs) {
@@ -1232,7 +1280,8 @@ int msig_batch_create(const msig_batch_job* jobs, int32_t njobs, const msig_tuni
   p->off_queue = align256(p->off_jobs + sizeof(KArgs) * p->jobs.size());
   p->off_fins = align256(p->off_queue + sizeof(WorkItem) * p->queue.size());
   p->off_finq = align256(p->off_fins + sizeof(FinArgs) * p->fins.size());
-  p->desc_bytes = align256(p->off_finq + sizeof(WorkItem) * p->finq.size());
+  p->off_ticket = align256(p->off_finq + sizeof(WorkItem) * p->finq.size());
+  p->desc_bytes = align256(p->off_ticket + sizeof(unsigned long long));
   *plan = p;
   return 0;
 }
@@ -1252,6 +1301,8 @@ int msig_batch_pack(const msig_batch* plan, void* host, size_t bytes, void* devi
   std::memcpy(h + plan->off_queue, plan->queue.data(), sizeof(WorkItem) * plan->queue.size());
   std::memcpy(h + plan->off_fins, plan->fins.data(), sizeof(FinArgs) * plan->fins.size());
   std::memcpy(h + plan->off_finq, plan->finq.data(), sizeof(WorkItem) * plan->finq.size());
+  const unsigned long long ticket0 = 0;  // the work counter starts at zero
+  std::memcpy(h + plan->off_ticket, &ticket0, sizeof ticket0);
   (void)device;  // the block holds no absolute pointers into itself
   return 0;
 }
@@ -1264,7 +1315,7 @@ int msig_batch_workspace_bytes(const msig_batch* plan, size_t* bytes) {
 
 int msig_batch_launches(const msig_batch* plan) {
   if (!plan) return fail(MSIG_EINVAL, "NULL plan");
-  return (int)plan->classes.size() + 1;
+  return (int)plan->classes.size() + (plan->single_nwork ? 1 : 0) + 1;
 }
 
 int msig_batch_run(const msig_batch* plan, const void* device, void* workspace, size_t ws_bytes,
@@ -1284,6 +1335,12 @@ int msig_batch_run(const msig_batch* plan, const void* device, void* workspace, 
   const FinArgs* fins = reinterpret_cast<const FinArgs*>(d + plan->off_fins);
   const WorkItem* finq = reinterpret_cast<const WorkItem*>(d + plan->off_finq);
   float* ws = static_cast<float*>(workspace);
+  if (plan->single_nwork) {
+    unsigned long long* ticket = reinterpret_cast<unsigned long long*>(
+        const_cast<char*>(d) + plan->off_ticket);
+    e = launch_mean_ragged(jobs, queue, plan->single_nwork, ticket, ws, plan->single_grid, s);
+    if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
+  }
   for (const auto& c : plan->classes) {
     e = dispatch_mean_batch(c.N, c.kk, c.cplx, c.half, c.sched, jobs, queue + c.qoff, c.nwork, ws,
                             c.grid, s);

@@ -237,15 +237,16 @@ __device__ __forceinline__ void side_copy(const float* src, float* dst, int64_t 
 }
 
 // Grouped mean over segments: per-bin sums of |Zx|^2 (PSD), conj(Zx) Zy
-// (CSD) or both spectra and the cross term (COH), written per block.
+// (CSD) or both spectra and the cross term (COH), written per block.  `tid`
+// is threadIdx.x (passed in so a kernel that runs several bodies can keep
+// each body's thread-index arithmetic inside that body).
 template <int N, int E, int KIND, bool CPLX, bool HALF, int S = 0>
 __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, cf* lds_all,
-                                          float* red, const cf* twl, float* ws) {
+                                          float* red, const cf* twl, float* ws, int tid) {
   using PL = Plan<N, E, S>;
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
   constexpr int NC = KIND == K_PSD ? 1 : (KIND == K_CSD ? 2 : 4);
-  const int tid = threadIdx.x;
   const int g = tid / TPF, t = tid % TPF;
   cf* lds = lds_all + g * PL::LDS;
   const int unit0 = (bir * GPB + g) * a.run;
@@ -534,7 +535,7 @@ __global__ __launch_bounds__(BT, (mean_waves<N, KIND, HALF, CPLX>())) void k_mea
     __syncthreads();
   }
   mean_body<N, E, KIND, CPLX, HALF>(a, blockIdx.x / a.bpr, blockIdx.x % a.bpr, lds_all, red, twl,
-                                    a.ws);
+                                    a.ws, threadIdx.x);
 }
 
 // Per-frame spectra.  Real input: the two packed segments are separated with
@@ -998,13 +999,57 @@ __global__ __launch_bounds__(BT, (batch_waves<N, KIND, HALF>())) void k_mean_bat
     __syncthreads();
     if constexpr (HALF) {
       mean_body<N, E, KIND, false, true, S>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl,
-                                         ws);
+                                         ws, threadIdx.x);
     } else {
       mean_body<N, E, KIND, CPLX, false, S>(a, item.blk / a.bpr, item.blk % a.bpr, lds_all, red, twl,
-                                         ws);
+                                         ws, threadIdx.x);
     }
     __syncthreads();
   }
+}
+
+// ---- one persistent launch for a whole ragged batch ----
+// The work items of every welch and csd job on real input with hop = N/2
+// (any N) share one kernel: each item names its job, the job its class
+// (N, kind), and a switch runs that class's mean body, the same block a
+// standalone launch would run.  Items are taken in queue order from a 64-bit
+// ticket counter: a run makes exactly nwork + gridDim.x fetches (every block
+// stops at its first ticket past the work), so ticket mod (nwork + gridDim.x)
+// is the run's own index and the counter never needs resetting.  LDS is one
+// pool sized for the largest class; a class change refills its twiddle table.
+// Classes are 2 log2(N / 16) + (kind == csd).
+constexpr int RAGGED_CLASSES = 18;
+constexpr int ragged_class(int N, int kind) {
+  int l = 0;
+  while ((16 << l) < N) ++l;
+  return 2 * l + (kind == K_CSD ? 1 : 0);
+}
+template <int N, int KIND>
+constexpr int ragged_pool() {
+  using PL = Plan<N, (N < 16 ? N : 16)>;
+  using TH = TwiddleHome<N, (N < 16 ? N : 16), 0, mean_twl<N, KIND, true>()>;
+  return (BT / PL::TPF) * PL::LDS + (TH::LDS ? TH::LDS_ELEMS : 0);
+}
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int RAGGED_POOL = cmax(cmax(ragged_pool<4096, K_CSD>(), ragged_pool<4096, K_PSD>()),
+                                 cmax(ragged_pool<2048, K_CSD>(), ragged_pool<1024, K_CSD>()));
+
+template <int N, int KIND>
+__device__ __forceinline__ void ragged_item(const KArgs& a, int blk, bool refill, cf* pool,
+                                            float* red, float* ws_base, int tid) {
+  constexpr int E = N < 16 ? N : 16;
+  using PL = Plan<N, E>;
+  using TH = TwiddleHome<N, E, 0, mean_twl<N, KIND, true>()>;
+  static_assert(ragged_pool<N, KIND>() <= RAGGED_POOL, "LDS pool too small");
+  cf* twl = pool + (BT / PL::TPF) * PL::LDS;
+  if constexpr (TH::LDS) {
+    if (refill) {
+      fill_twiddle_lds<N, E>(twl, a.twt);
+      __syncthreads();
+    }
+  }
+  mean_body<N, E, KIND, false, true>(a, blk / a.bpr, blk % a.bpr, pool, red, twl,
+                                     ws_base + a.ws_off, tid);
 }
 
 }  // namespace msig

@@ -106,6 +106,11 @@ inline Geo geometry_mfma(int64_t rows, int64_t nseg, int maxrun, int64_t target_
 
 hipError_t launch_mean_mfma256(int prec, const KArgs& a, int64_t rows, hipStream_t s);
 
+// The single-launch ragged batch kernel (msig_ragged.hip).
+int ragged_blocks_per_cu();
+hipError_t launch_mean_ragged(const KArgs* jobs, const WorkItem* queue, int32_t nwork,
+                              unsigned long long* ticket, float* ws, int grid, hipStream_t s);
+
 // One pair of launchers per fused FFT size, defined in msig_n<N>.hip.
 #define MSIG_DECLARE_LAUNCHERS(NV)                                                         \
   hipError_t launch_mean_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows,  \

@@ -32,6 +32,7 @@ struct KArgs {
   const float* side_src;  // side copy, grid-strided over all blocks: side_len floats to side_dst
   float* side_dst;
   int64_t side_len;
+  int32_t cls;       // single-launch batch plans: the job's (N, kind) class
 };
 
 // Raw samples of consecutive units of one row, fetched one unit ahead so the

@@ -17,7 +17,7 @@ import os
 
 # MSIG_LIBRARY selects another build of the library (e.g. a tuning variant)
 LIB_PATH = Path(os.environ.get("MSIG_LIBRARY") or Path(__file__).with_name("libmsig.so"))
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 MSIG_EINVAL = -22
 MSIG_ENOTSUP = -95
@@ -82,6 +82,7 @@ class MsigTuning(ctypes.Structure):
         ("target_blocks", ctypes.c_int32 * 13),
         ("blocks_per_cu", ctypes.c_int32 * 13),
         ("radix_schedule", ctypes.c_int32 * 13),
+        ("per_class_launches", ctypes.c_int32),
     ]
 
 

@@ -4,8 +4,9 @@ No counterpart in the reference, which launches one Metal kernel per call
 (src/mlx_signal_processing/_stft_metal.py:530-557, 703-736).  Here many
 requests of uneven channel count, length and nperseg are planned once
 (``msig_batch_create``), and each run is kernels only: one persistent launch
-per FFT-size class walking that class's ragged work queue, then one finalize
-launch for every request (``msig_batch_run``).  A run allocates nothing and
+walks the ragged work queue of every welch and csd request at hop = nfft/2
+(all FFT sizes), one per (nfft, kind) class covers any other request, and one
+finalize launch completes every request (``msig_batch_run``).  A run allocates nothing and
 copies nothing from the host, so it can be captured in a HIP graph
 (:meth:`SpectralBatch.graph`) and replayed.
 
@@ -48,13 +49,14 @@ class SpectralRequest:
     scaling: str = "density"
 
 
-def tuning_struct(tuning) -> _hip.MsigTuning | None:
+def tuning_struct(tuning, per_class_launches: bool = False) -> _hip.MsigTuning | None:
     """{nfft: {"max_run", "target_blocks", "blocks_per_cu", "radix_schedule"}}
-    -> MsigTuning."""
-    if not tuning:
+    -> MsigTuning (None for the defaults)."""
+    if not tuning and not per_class_launches:
         return None
     t = _hip.MsigTuning()
-    for nfft, knobs in tuning.items():
+    t.per_class_launches = int(bool(per_class_launches))
+    for nfft, knobs in (tuning or {}).items():
         l2 = int(nfft).bit_length() - 1
         t.max_run[l2] = int(knobs.get("max_run", 0))
         t.target_blocks[l2] = int(knobs.get("target_blocks", 0))
@@ -69,9 +71,14 @@ _KINDS = {"welch": _hip.PSD_MEAN, "csd": _hip.CSD_MEAN, "coherence": _hip.COHERE
 class SpectralBatch:
     """A planned ragged batch.  ``run()`` launches it on the current stream;
     ``outputs[i]`` is request i's result tensor ((..., bins)), rewritten by
-    every run; ``freqs[i]`` its frequency grid (the batch's own copy)."""
+    every run; ``freqs[i]`` its frequency grid (the batch's own copy).
 
-    def __init__(self, requests, tuning=None):
+    Welch and csd requests with noverlap = nperseg/2 = nfft/2 all run in one
+    persistent launch; ``per_class_launches=True`` gives them one launch per
+    (nfft, kind) class instead (the results are the same bit for bit).  Runs
+    of one batch must not overlap: they share its workspace and work counter."""
+
+    def __init__(self, requests, tuning=None, per_class_launches: bool = False):
         self.device = compute_device()
         lib = _hip.load()
         n = len(requests)
@@ -87,7 +94,7 @@ class SpectralBatch:
         for i, r in enumerate(requests):
             jobs[i] = self._plan(r)
         self.jobs = jobs  # the planned msig_batch_job array (kept alive with its tensors)
-        self._tuning = tuning_struct(tuning)
+        self._tuning = tuning_struct(tuning, per_class_launches)
         handle = ctypes.c_void_p()
         tp = ctypes.byref(self._tuning) if self._tuning is not None else None
         _hip.check(lib.msig_batch_create(jobs, n, tp, ctypes.byref(handle)), "batch plan")

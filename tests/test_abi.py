@@ -81,9 +81,9 @@ def test_tuning_struct_matches_header():
 
     text = (ROOT / "include" / "msig.h").read_text()
     body = re.search(r"typedef struct msig_tuning \{(.*?)\} msig_tuning;", text, re.S).group(1)
-    fields = re.findall(r"int32_t (\w+)\[13\];", body)
-    assert fields == [f for f, _ in _hip.MsigTuning._fields_]
-    assert ctypes.sizeof(_hip.MsigTuning) == 4 * 13 * len(fields)
+    fields = re.findall(r"int32_t (\w+)(\[13\])?;", body)
+    assert [f for f, _ in fields] == [f for f, _ in _hip.MsigTuning._fields_]
+    assert ctypes.sizeof(_hip.MsigTuning) == sum(4 * 13 if arr else 4 for _, arr in fields)
     abi = int(re.search(r"#define MSIG_ABI_VERSION (\d+)", text).group(1))
     assert abi == _hip.ABI_VERSION
 

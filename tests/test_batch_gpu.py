@@ -148,3 +148,100 @@ def test_native_autotune_times_every_candidate(rng):
     for out, (kind, x, y, kw) in zip(b.outputs, host):
         assert_peak_close(out, _oracle(kind, x, y, kw), TOL)
     b.close()
+
+
+def _half_requests(rng, dev, sizes=(16, 32, 64, 128, 256, 512, 1024, 2048, 4096)):
+    """welch and csd at hop = nperseg/2 over every fused size: the jobs of the
+    single persistent launch (odd and even segment counts, 1-3 rows)."""
+    reqs, host = [], []
+    for i, nps in enumerate(sizes):
+        for kind in ("welch", "csd"):
+            ch = 1 + (i + len(kind)) % 3
+            n = nps * int(rng.integers(1, 12)) + int(rng.integers(0, nps))
+            win = "hann" if (i % 2) else "blackman"
+            x = make_sig(rng, (ch, n))
+            y = (0.5 * x + make_sig(rng, (ch, n))).astype(np.float32) if kind == "csd" else None
+            kw = dict(fs=1000.0, window=win, nperseg=nps, detrend="constant" if i % 3 else False)
+            host.append((kind, x, y, kw))
+            reqs.append(SpectralRequest(kind, torch.from_numpy(x).to(dev),
+                                        None if y is None else torch.from_numpy(y).to(dev), **kw))
+    return reqs, host
+
+
+def test_single_launch_matches_per_class_and_oracle(rng):
+    """All welch / csd jobs at hop = nfft/2 run in one persistent launch (plus
+    the finalize): bitwise the per-class launches' results, and the oracle's
+    to 1e-5 of the peak, over all nine fused sizes."""
+    dev = torch.device("cuda")
+    reqs, host = _half_requests(rng, dev)
+    one = SpectralBatch(reqs)
+    assert one.launches == 2
+    per = SpectralBatch(reqs, per_class_launches=True)
+    assert per.launches == 2 * 9 + 1
+    a = [o.clone() for o in one.run()]
+    b = [o.clone() for o in per.run()]
+    torch.cuda.synchronize()
+    for i, ((kind, x, y, kw), oa, ob) in enumerate(zip(host, a, b)):
+        assert torch.equal(oa, ob), (i, kind, kw["nperseg"])
+        assert_peak_close(oa, _oracle(kind, x, y, kw), TOL)
+
+
+def test_single_launch_mixed_with_other_classes(rng):
+    """A batch with coherence, other hops and a radix-1 size keeps those in
+    per-class launches beside the single launch; every result is the oracle's
+    and bitwise the per-class plan's."""
+    dev = torch.device("cuda")
+    reqs, host = _requests(rng, dev)
+    tuning = {512: {"radix_schedule": 1}}
+    one = SpectralBatch(reqs, tuning=tuning)
+    per = SpectralBatch(reqs, tuning=tuning, per_class_launches=True)
+    # single: welch/csd 256, 1024, 4096, 64 at hop = nfft/2 share one launch
+    assert one.launches < per.launches
+    a = [o.clone() for o in one.run()]
+    b = [o.clone() for o in per.run()]
+    for i, ((kind, x, y, kw), oa, ob) in enumerate(zip(host, a, b)):
+        assert torch.equal(oa, ob), i
+        assert_peak_close(oa, _oracle(kind, x, y, kw), TOL)
+
+
+def test_single_launch_repeated_runs_and_two_plans(rng):
+    """The work counter advances across runs without a reset: five runs of one
+    plan, interleaved with runs of a second plan, give the same bits."""
+    dev = torch.device("cuda")
+    reqs, _ = _half_requests(rng, dev, sizes=(256, 1024, 4096))
+    p1 = SpectralBatch(reqs)
+    p2 = SpectralBatch(reqs[::-1])
+    first = [o.clone() for o in p1.run()]
+    second = [o.clone() for o in p2.run()]
+    for rep in range(5):
+        for o in p1.outputs + p2.outputs:
+            o.zero_()
+        p1.run()
+        p2.run()
+        torch.cuda.synchronize()
+        for i, o in enumerate(p1.outputs):
+            assert torch.equal(o, first[i]), (rep, i)
+        for i, o in enumerate(p2.outputs):
+            assert torch.equal(o, second[i]), (rep, i)
+    for i in range(len(reqs)):
+        assert torch.equal(first[i], second[len(reqs) - 1 - i]), i
+
+
+def test_single_launch_autotune(rng):
+    """Tuning for the single launch times tile shapes at radix schedule 0 only,
+    and the winner keeps every job in the one launch (2 launches a run)."""
+    from mlx_signal_amd.tuning import SINGLE_CANDIDATES, candidates_for, time_candidates
+
+    dev = torch.device("cuda")
+    reqs, host = _half_requests(rng, dev, sizes=(256,))
+    cands = candidates_for(256, per_class_launches=False)
+    assert len(cands) == len(SINGLE_CANDIDATES)
+    assert all(c["radix_schedule"] == 0 for c in cands)
+    ms, best = time_candidates(reqs, 256, cands, reps=2, per_class_launches=False)
+    assert len(ms) == len(cands) and all(t > 0 for t in ms)
+    b = SpectralBatch(reqs, tuning={256: cands[best]})
+    assert b.launches == 2
+    b.run()
+    for out, (kind, x, y, kw) in zip(b.outputs, host):
+        assert_peak_close(out, _oracle(kind, x, y, kw), TOL)
+    b.close()
