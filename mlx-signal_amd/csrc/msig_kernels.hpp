@@ -174,7 +174,9 @@ __device__ __forceinline__ void fill_twiddle_lds(cf* twl, const cf* table) {
 // Forward FFT of the group's N points held E per thread (register e holds
 // sample t + TPF e); tw is this thread's twiddle registers (TWL = false) or
 // the block's shared LDS table (TWL = true); radix schedule S (Plan).
-template <int N, int E, bool TWL, int P = 1, int S = 0>
+// SPLIT: each exchange moves the real parts, then the imaginary parts,
+// through Plan::LDS floats at `lds` (half the LDS of a complex exchange).
+template <int N, int E, bool TWL, int P = 1, int S = 0, bool SPLIT = false>
 __device__ __forceinline__ void fft_passes(int t, cf* v, const cf* tw, cf* lds) {
   using PL = Plan<N, E, S>;
   if constexpr (P == 1) {
@@ -188,12 +190,23 @@ __device__ __forceinline__ void fft_passes(int t, cf* v, const cf* tw, cf* lds) 
     pass_compute<N, E, 0, TWL, S>(v, tw, t);
   }
   if constexpr (P < PL::NP) {
-    pass_store<N, E, P - 1, S>(t, v, lds);
-    group_sync<PL::TPF>();
-    pass_load<N, E, P, S>(t, v, lds);
+    if constexpr (SPLIT) {
+      float* lf = reinterpret_cast<float*>(lds);
+      pass_store_part<N, E, P - 1, S, 0>(t, v, lf);
+      group_sync<PL::TPF>();
+      pass_load_part<N, E, P, S, 0>(t, v, lf);
+      group_sync<PL::TPF>();  // real reads done before the imaginary stores
+      pass_store_part<N, E, P - 1, S, 1>(t, v, lf);
+      group_sync<PL::TPF>();
+      pass_load_part<N, E, P, S, 1>(t, v, lf);
+    } else {
+      pass_store<N, E, P - 1, S>(t, v, lds);
+      group_sync<PL::TPF>();
+      pass_load<N, E, P, S>(t, v, lds);
+    }
     if constexpr (PL::TPF > 64) __syncthreads();  // reads done before next stores
     pass_compute<N, E, P, TWL, S>(v, tw, t);
-    fft_passes<N, E, TWL, P + 1, S>(t, v, tw, lds);
+    fft_passes<N, E, TWL, P + 1, S, SPLIT>(t, v, tw, lds);
   }
 }
 
@@ -236,6 +249,19 @@ __device__ __forceinline__ void side_copy(const float* src, float* dst, int64_t 
     as_global_mut(dst)[i] = as_global(src)[i];
 }
 
+// Mean kernels whose FFT exchanges go through LDS one component at a time
+// (fft_passes SPLIT): the block's exchange buffers then take half the LDS,
+// which with the twiddle table lets four blocks share a CU.
+template <int N, int KIND, bool HALF, bool CPLX = false>
+constexpr bool mean_split() {
+  return KIND == K_PSD && HALF && !CPLX && N == 1024;
+}
+template <int N, int KIND, bool HALF, bool CPLX = false>
+constexpr int mean_lds_cf() {
+  using PL = Plan<N, (N < 16 ? N : 16)>;
+  return (BT / PL::TPF) * (mean_split<N, KIND, HALF, CPLX>() ? PL::LDS / 2 : PL::LDS);
+}
+
 // Grouped mean over segments: per-bin sums of |Zx|^2 (PSD), conj(Zx) Zy
 // (CSD) or both spectra and the cross term (COH), written per block.  `tid`
 // is threadIdx.x (passed in so a kernel that runs several bodies can keep
@@ -247,8 +273,10 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
   constexpr int NC = KIND == K_PSD ? 1 : (KIND == K_CSD ? 2 : 4);
+  constexpr bool SPLIT = mean_split<N, KIND, HALF, CPLX>();
+  static_assert(!SPLIT || PL::LDS % 2 == 0, "split exchange needs an even slot count");
   const int g = tid / TPF, t = tid % TPF;
-  cf* lds = lds_all + g * PL::LDS;
+  cf* lds = lds_all + g * (SPLIT ? PL::LDS / 2 : PL::LDS);
   const int unit0 = (bir * GPB + g) * a.run;
 
   using TH = TwiddleHome<N, E, S, mean_twl<N, KIND, HALF>()>;
@@ -286,16 +314,16 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
   // signal's windowed points are live at a time.
   auto transform_acc = [&](cf (&zx)[E], cf (&zy)[E], bool active, auto&& prep_y) {
     if constexpr (KIND == K_PSD) {
-      fft_passes<N, E, TH::LDS, 1, S>(t, zx, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S, SPLIT>(t, zx, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) acc[0][e] = fmaf(zx[e].im, zx[e].im, fmaf(zx[e].re, zx[e].re, acc[0][e]));
       }
     } else {
-      fft_passes<N, E, TH::LDS, 1, S>(t, zx, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S, SPLIT>(t, zx, tw, lds);
       group_sync<TPF>();  // last-pass reads of zx done before zy's stores
       prep_y();
-      fft_passes<N, E, TH::LDS, 1, S>(t, zy, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S, SPLIT>(t, zy, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -428,7 +456,7 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
     if constexpr (KIND == K_PSD) {
       if (more) sx.fetch(unit + 1, false, a);
       prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::LDS, 1, S>(t, zx, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S, SPLIT>(t, zx, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) acc[0][e] = fmaf(zx[e].im, zx[e].im, fmaf(zx[e].re, zx[e].re, acc[0][e]));
@@ -441,10 +469,10 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
         sy.fetch(unit + 1, false, a);
       }
       prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::LDS, 1, S>(t, zx, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S, SPLIT>(t, zx, tw, lds);
       group_sync<TPF>();  // last-pass reads of zx done before zy's stores
       prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
-      fft_passes<N, E, TH::LDS, 1, S>(t, zy, tw, lds);
+      fft_passes<N, E, TH::LDS, 1, S, SPLIT>(t, zy, tw, lds);
       if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -508,6 +536,7 @@ constexpr int mean_waves() {
     return HALF ? 2 : 1;
   }
   if constexpr (KIND == K_CSD && !HALF && !CPLX && N >= 512) return 1;
+  if constexpr (mean_split<N, KIND, HALF, CPLX>()) return 4;
   return KIND == K_PSD ? 3 : (KIND == K_CSD ? 2 : (HALF ? 2 : 1));
 }
 // The extension stream at N = 1024 spilled at three waves per SIMD; two run
@@ -527,7 +556,7 @@ template <int N, int E, int KIND, bool CPLX, bool HALF>
 __global__ __launch_bounds__(BT, (mean_waves<N, KIND, HALF, CPLX>())) void k_mean(const KArgs a) {
   using PL = Plan<N, E>;
   using TH = TwiddleHome<N, E, 0, mean_twl<N, KIND, HALF>()>;
-  __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
+  __shared__ cf lds_all[mean_lds_cf<N, KIND, HALF, CPLX>()];
   __shared__ float red[2 * (BT / 64)];
   __shared__ cf twl[TH::LDS_ELEMS];
   if constexpr (TH::LDS) {
@@ -980,9 +1009,8 @@ template <int N, int E, int KIND, bool CPLX, bool HALF, int S>
 __global__ __launch_bounds__(BT, (batch_waves<N, KIND, HALF>())) void k_mean_batch(
     const KArgs* __restrict__ jobs, const WorkItem* __restrict__ queue, int32_t nwork,
     float* ws_base) {
-  using PL = Plan<N, E>;
   using TH = TwiddleHome<N, E, S, mean_twl<N, KIND, HALF>()>;
-  __shared__ cf lds_all[(BT / PL::TPF) * PL::LDS];
+  __shared__ cf lds_all[mean_lds_cf<N, KIND, HALF, CPLX>()];
   __shared__ float red[2 * (BT / 64)];
   __shared__ cf twl[TH::LDS_ELEMS];
   bool filled = false;

@@ -47,7 +47,14 @@ static bool run_pass(std::vector<std::vector<cf>>& v, const std::vector<std::vec
     std::fill(wcount.begin(), wcount.end(), 0);
     for (int t = 0; t < TPF; ++t) {
       msig::pass_store<N, E, P - 1, S>(t, v[t].data(), lds.data());
-      for (int e = 0; e < E; ++e) wcount[PL::pad(PL::template out_pos<P - 1>(t, e))]++;
+      for (int e = 0; e < E; ++e) {
+        const int slot = PL::pad(PL::template out_pos<P - 1>(t, e));
+        if (msig::out_slot<N, E, P - 1, S>(t, e) != slot) {
+          std::printf("N=%d pass %d: split store slot differs at t=%d e=%d\n", N, P - 1, t, e);
+          return false;
+        }
+        wcount[slot]++;
+      }
     }
     for (int i = 0; i < (int)wcount.size(); ++i)
       if (wcount[i] > 1) {
@@ -57,6 +64,10 @@ static bool run_pass(std::vector<std::vector<cf>>& v, const std::vector<std::vec
     for (int t = 0; t < TPF; ++t) {
       for (int e = 0; e < E; ++e) {
         const int slot = PL::pad(PL::template in_pos<P>(t, e));
+        if (msig::in_slot<N, E, P, S>(t, e) != slot) {
+          std::printf("N=%d pass %d: split load slot differs at t=%d e=%d\n", N, P, t, e);
+          return false;
+        }
         if (wcount[slot] != 1) {
           std::printf("N=%d pass %d: read of unwritten slot %d\n", N, P, slot);
           return false;
