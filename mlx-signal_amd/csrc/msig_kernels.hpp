@@ -251,10 +251,12 @@ __device__ __forceinline__ void side_copy(const float* src, float* dst, int64_t 
 
 // Mean kernels whose FFT exchanges go through LDS one component at a time
 // (fft_passes SPLIT): the block's exchange buffers then take half the LDS,
-// which with the twiddle table lets four blocks share a CU.
+// which with the twiddle table lets four blocks share a CU.  The hop == N/2
+// welch kernels up to N = 1024 fit four waves' VGPRs without spills this way
+// (119-125); at 2048 and 4096 they need more (133, 171).
 template <int N, int KIND, bool HALF, bool CPLX = false>
 constexpr bool mean_split() {
-  return KIND == K_PSD && HALF && !CPLX && N == 1024;
+  return KIND == K_PSD && HALF && !CPLX && N >= 32 && N <= 1024;
 }
 template <int N, int KIND, bool HALF, bool CPLX = false>
 constexpr int mean_lds_cf() {
