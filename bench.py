@@ -100,32 +100,33 @@ def warm_up(torch, fn, bufs, warmup, min_ms=WARM_MS):
 
 
 def time_steps(torch, fn, bufs, steps, warmup, kernel_timing=False):
-    """Wall clock over K steps (synchronized both sides) plus per-step HIP event
-    durations on the current stream (the stream the kernels launch on), after
-    warm_up().  With kernel_timing, the library also records HIP events on that
-    stream around each step's fused transform kernel; returns its mean
-    duration too."""
+    """Time K back-to-back steps after warm_up(), synchronized on both sides:
+    (wall ms per step, HIP-event ms per step over the whole region on the
+    current stream, which is the stream the kernels launch on, None, fused
+    kernel ms).  With kernel_timing, the library also records HIP events on
+    that stream around each step's fused transform kernel and its mean
+    duration is returned; no other event sits between the steps."""
     from mlx_signal_amd import _kernels
 
     warm_up(torch, fn, bufs, warmup)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(steps)]
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
     if kernel_timing:
         _kernels.kernel_timing_arm(steps)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
+    e0.record()
     for i in range(steps):
-        evs[i][0].record()
         fn(bufs[i % len(bufs)])
-        evs[i][1].record()
+    e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ev_ms = [a.elapsed_time(b) for a, b in evs]
     kernel_ms = None
     if kernel_timing:
         tot, n = _kernels.kernel_timing_read()
         assert n == steps, f"timed {n} fused kernels over {steps} steps"
         kernel_ms = tot / n
-    return wall / steps * 1e3, statistics.median(ev_ms), ev_ms, kernel_ms
+    return wall / steps * 1e3, e0.elapsed_time(e1) / steps, None, kernel_ms
 
 
 TRAFFIC_PROFILE = "profiles/welch_64x2p20_traffic.json"
