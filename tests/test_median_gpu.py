@@ -18,7 +18,8 @@ def _np_median(a, scale=1.0):
     return (np.median(a.astype(np.float64), axis=1) * scale).astype(np.float32)
 
 
-@pytest.mark.parametrize("nseg", [1, 2, 3, 4, 7, 64, 255, 256, 1023, 2047, 2048, 4097, 16000])
+@pytest.mark.parametrize("nseg", [1, 2, 3, 4, 7, 64, 255, 256, 512, 513, 1023, 1024, 1025, 2047, 2048,
+                                  2049, 4097, 8192, 8193, 16000])
 def test_median_kernel_exact_random(rng, nseg):
     rows, bins = 3, 37
     a = rng.standard_normal((rows, nseg, bins)).astype(np.float32)
