@@ -103,16 +103,16 @@ def time_steps(torch, fn, bufs, steps, warmup, kernel_timing=False):
     """Time K back-to-back steps after warm_up(), synchronized on both sides:
     (wall ms per step, HIP-event ms per step over the whole region on the
     current stream, which is the stream the kernels launch on, None, fused
-    kernel ms).  With kernel_timing, the library also records HIP events on
-    that stream around each step's fused transform kernel and its mean
-    duration is returned; no other event sits between the steps."""
+    kernel ms).  Nothing else is recorded inside that region.  With
+    kernel_timing, a second region of the same K steps follows at once, in
+    which the library hands a HIP event pair to each step's fused transform
+    launch (hipExtLaunchKernel stamps the pair with that dispatch's start and
+    end); the mean of those K durations is the fused kernel ms."""
     from mlx_signal_amd import _kernels
 
     warm_up(torch, fn, bufs, warmup)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    if kernel_timing:
-        _kernels.kernel_timing_arm(steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0.record()
@@ -123,6 +123,10 @@ def time_steps(torch, fn, bufs, steps, warmup, kernel_timing=False):
     wall = time.perf_counter() - t0
     kernel_ms = None
     if kernel_timing:
+        _kernels.kernel_timing_arm(steps)
+        for i in range(steps):
+            fn(bufs[i % len(bufs)])
+        torch.cuda.synchronize()
         tot, n = _kernels.kernel_timing_read()
         assert n == steps, f"timed {n} fused kernels over {steps} steps"
         kernel_ms = tot / n

@@ -343,9 +343,10 @@ int msig_median_frames(const float* frames, int64_t rows, int32_t nseg, int32_t 
 /* Kernel timing (instrumentation; no counterpart in the reference, which
  * times whole calls).  After msig_kernel_timing_arm(k) on a device, the next
  * k calls of msig_spectral_mean / msig_spectral_frames (and the operator-level
- * entry points built on them) on that device record a HIP event pair on their
- * stream directly around their fused transform kernel, excluding the
- * finalize.  msig_kernel_timing_read waits for the recorded pairs, returns
+ * entry points built on them) on that device hand a HIP event pair to the
+ * launch of their fused transform kernel (hipExtLaunchKernel), which stamps
+ * the pair with that dispatch's start and end on the call's stream; the
+ * finalize is excluded.  msig_kernel_timing_read waits for the recorded pairs, returns
  * their summed elapsed milliseconds and count, and disarms.  k = 0 disarms.
  * Not for use under graph capture. */
 int msig_kernel_timing_arm(int32_t max_launches);

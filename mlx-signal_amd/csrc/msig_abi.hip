@@ -181,17 +181,18 @@ static std::pair<hipEvent_t, hipEvent_t>* timing_slot() {
   return &g_kt.ev[g_kt.used++];
 }
 
-// Launch `body` between the two events of a timing slot, if one is armed.
+thread_local LaunchTiming g_launch_timing;
+
+// Run `body` (which makes one fused transform launch) with the events of a
+// timing slot, if one is armed, handed to that launch (launch_timed).
 template <class F>
-static hipError_t timed_launch(hipStream_t s, F&& body) {
+static hipError_t timed_launch(hipStream_t, F&& body) {
   auto* slot = timing_slot();
-  if (slot) {
-    hipError_t e = hipEventRecord(slot->first, s);
-    if (e != hipSuccess) return e;
-  }
+  if (!slot) return body();
+  g_launch_timing = LaunchTiming{slot->first, slot->second};
   hipError_t e = body();
-  if (e != hipSuccess || !slot) return e;
-  return hipEventRecord(slot->second, s);
+  g_launch_timing = LaunchTiming{};
+  return e;
 }
 
 // ---------------------------------------------------------------- finalize

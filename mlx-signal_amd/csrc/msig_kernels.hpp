@@ -264,13 +264,22 @@ constexpr int mean_lds_cf() {
   return (BT / PL::TPF) * (mean_split<N, KIND, HALF, CPLX>() ? PL::LDS / 2 : PL::LDS);
 }
 
+struct NoPrologue {
+  __device__ void operator()() const {}
+};
+
 // Grouped mean over segments: per-bin sums of |Zx|^2 (PSD), conj(Zx) Zy
 // (CSD) or both spectra and the cross term (COH), written per block.  `tid`
 // is threadIdx.x (passed in so a kernel that runs several bodies can keep
 // each body's thread-index arithmetic inside that body).
-template <int N, int E, int KIND, bool CPLX, bool HALF, int S = 0>
+// `prologue` runs once in every thread before the twiddle table is read (k_mean
+// fills the block's table there).  In the hop == N/2 stream it runs right
+// after the group's first loads are issued, so that the fill and its barrier
+// overlap those HBM loads instead of preceding them.
+template <int N, int E, int KIND, bool CPLX, bool HALF, int S = 0, class Pro = NoPrologue>
 __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, cf* lds_all,
-                                          float* red, const cf* twl, float* ws, int tid) {
+                                          float* red, const cf* twl, float* ws, int tid,
+                                          Pro&& prologue = Pro{}) {
   using PL = Plan<N, E, S>;
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
@@ -369,11 +378,13 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
     const bool has_full = unit0 < uf;
     const bool has_tail = full >= unit0 && full < u1;
     const float inv = 1.0f / (float)a.nperseg;
+    const float dinv = a.detrend ? inv : 0.f;  // zero means without detrend
     float sxc = 0.f, syc = 0.f;  // group sums of the carried half (detrend)
     if (has_full) {
       sx.fetch_full(unit0, true);
       if constexpr (KIND != K_PSD) sy.fetch_full(unit0, true);
     }
+    prologue();
     if (a.detrend) {
       if (has_full) {
         sxc = sx.first_half_sum();
@@ -382,14 +393,22 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
       group_sum2<TPF>(sxc, syc, red);  // every group: the same barriers (see below)
     }
     // segment means of the fetched unit of one signal (detrend)
+    // Single-signal groups within one wave (TPF <= 64) run the sums whether
+    // or not they detrend (means of zero without), which keeps the loop one
+    // basic block (welch 64 x 2^20, nperseg 1024: 88.6 -> 87.6 us per
+    // call).  The two-signal kinds keep the test: branch-free they ran 6-8%
+    // slower at N = 1024.  Groups of several waves synchronise in
+    // group_sum2 and must keep the test, so that their barrier sequence
+    // matches prep_unit's.
+    constexpr bool DALL = TPF <= 64 && KIND == K_PSD;
     auto means = [&](const UnitStream<N, E, CPLX, HALF>& st, float& carry, float& mr, float& mi) {
       mr = 0.f;
       mi = 0.f;
-      if (a.detrend) {
+      if (DALL || a.detrend) {
         float sb, sc;
         st.new_half_sums(sb, sc);
         group_sum2<TPF>(sb, sc, red);
-        st.half_means(carry, sb, sc, inv, mr, mi);
+        st.half_means(carry, sb, sc, dinv, mr, mi);
       }
     };
     if (has_full) {
@@ -398,7 +417,7 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
         cf zx[E], zy[E];
         float mrx, mix;
         means(sx, sxc, mrx, mix);
-        sx.take_full_prepped(zx, w, a.detrend, mrx, mix);
+        sx.take_full_prepped(zx, w, DALL || a.detrend, mrx, mix);
         sx.fetch_full(nxt, false);
         if constexpr (KIND == K_PSD) {
           transform_acc(zx, zy, true, [] {});
@@ -409,7 +428,7 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
           transform_acc(zx, zy, true, [&] {
             float mry, miy;
             means(sy, syc, mry, miy);
-            sy.take_full_prepped(zy, w, a.detrend, mry, miy);
+            sy.take_full_prepped(zy, w, DALL || a.detrend, mry, miy);
             sy.fetch_full(nxt, false);
           });
         }
@@ -447,6 +466,7 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
       }
     }
   } else {
+  prologue();  // before the loads here: the general stream's first unit is a full segment set
   sx.fetch(unit0, true, a);
   if constexpr (KIND != K_PSD) sy.fetch(unit0, true, a);
   for (int it = 0; it < a.run; ++it) {
@@ -561,12 +581,13 @@ __global__ __launch_bounds__(BT, (mean_waves<N, KIND, HALF, CPLX>())) void k_mea
   __shared__ cf lds_all[mean_lds_cf<N, KIND, HALF, CPLX>()];
   __shared__ float red[2 * (BT / 64)];
   __shared__ cf twl[TH::LDS_ELEMS];
-  if constexpr (TH::LDS) {
-    fill_twiddle_lds<N, E>(twl, a.twt);
-    __syncthreads();
-  }
   mean_body<N, E, KIND, CPLX, HALF>(a, blockIdx.x / a.bpr, blockIdx.x % a.bpr, lds_all, red, twl,
-                                    a.ws, threadIdx.x);
+                                    a.ws, threadIdx.x, [&] {
+                                      if constexpr (TH::LDS) {
+                                        fill_twiddle_lds<N, E>(twl, a.twt);
+                                        __syncthreads();
+                                      }
+                                    });
 }
 
 // Per-frame spectra.  Real input: the two packed segments are separated with

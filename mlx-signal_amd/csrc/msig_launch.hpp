@@ -5,9 +5,35 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <tuple>
+#include <utility>
+
 #include "msig_kernels.hpp"
 
 namespace msig {
+
+// Events that time the next fused transform launch made on this thread (the
+// kernel-timing instrumentation; both null otherwise).  launch_timed passes
+// them to hipExtLaunchKernel, which stamps them with the dispatch's own start
+// and end, so the measured window holds the kernel and nothing queued
+// around it.
+struct LaunchTiming {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchTiming g_launch_timing;
+
+template <class... P, class... A>
+hipError_t launch_timed(void (*k)(P...), dim3 grid, dim3 block, hipStream_t s, A&&... args) {
+  static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+  auto held = std::tuple<P...>(std::forward<A>(args)...);
+  void* argv[sizeof...(P) > 0 ? sizeof...(P) : 1];
+  std::apply([&](auto&... v) {
+    int i = 0;
+    ((argv[i++] = static_cast<void*>(&v)), ...);
+  }, held);
+  return hipExtLaunchKernel(reinterpret_cast<const void*>(k), grid, block, argv, 0, s,
+                            g_launch_timing.start, g_launch_timing.stop, 0);
+}
 
 // Segment-run decomposition of one job.  Depends only on the job's geometry,
 // so a job's reduction order (hence its bits) is the same whether it runs
@@ -139,15 +165,15 @@ MSIG_DECLARE_LAUNCHERS(4096)
 // instantiates the kernels for FFT size N.  `half` selects the hop == N/2
 // register-reuse kernels (real input, nperseg == N).
 #define MSIG_LAUNCH_MEAN(NV, K, C, H) \
-  hipLaunchKernelGGL((k_mean<NV, (NV < 16 ? NV : 16), K, C, H>), grid, block, 0, s, a)
+  le = launch_timed(&k_mean<NV, (NV < 16 ? NV : 16), K, C, H>, grid, block, s, a)
 #define MSIG_LAUNCH_FRAMES(NV, K, C, H) \
-  hipLaunchKernelGGL((k_frames<NV, (NV < 16 ? NV : 16), K, C, H>), grid, block, 0, s, a)
+  le = launch_timed(&k_frames<NV, (NV < 16 ? NV : 16), K, C, H>, grid, block, s, a)
 // complex frames through the virtual zero extension (ext) or plain
 #define MSIG_LAUNCH_FRAMES_COMPLEX(NV, C, H)                                                    \
   do {                                                                                        \
     if (ext)                                                                                  \
-      hipLaunchKernelGGL((k_frames<NV, (NV < 16 ? NV : 16), F_COMPLEX, C, H, true>), grid, block, \
-                         0, s, a);                                                            \
+      le = launch_timed(&k_frames<NV, (NV < 16 ? NV : 16), F_COMPLEX, C, H, true>, grid, block, \
+                        s, a);                                                                \
     else                                                                                      \
       MSIG_LAUNCH_FRAMES(NV, F_COMPLEX, C, H);                                                \
   } while (0)
@@ -195,6 +221,7 @@ MSIG_DECLARE_LAUNCHERS(4096)
   hipError_t launch_mean_##NV(int kind, bool cplx, bool half, const KArgs& a, int64_t rows, \
                               hipStream_t s) {                                             \
     const dim3 grid((unsigned)(rows * a.bpr)), block(BT);                                  \
+    hipError_t le = hipSuccess;                                                            \
     if (cplx) {                                                                            \
       MSIG_KIND_SWITCH(MSIG_LAUNCH_MEAN, NV, K_PSD, K_CSD, K_COH, true, false)             \
     } else if (half) {                                                                     \
@@ -202,11 +229,12 @@ MSIG_DECLARE_LAUNCHERS(4096)
     } else {                                                                               \
       MSIG_KIND_SWITCH(MSIG_LAUNCH_MEAN, NV, K_PSD, K_CSD, K_COH, false, false)            \
     }                                                                                      \
-    return hipGetLastError();                                                              \
+    return le != hipSuccess ? le : hipGetLastError();                                      \
   }                                                                                        \
   hipError_t launch_frames_##NV(int kind, bool cplx, bool half, bool ext, const KArgs& a,  \
                                 int64_t rows, hipStream_t s) {                             \
     const dim3 grid((unsigned)(rows * a.bpr)), block(BT);                                  \
+    hipError_t le = hipSuccess;                                                            \
     if (cplx) {                                                                            \
       if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES_COMPLEX(NV, true, false);                  \
       else MSIG_KIND_SWITCH(MSIG_LAUNCH_FRAMES, NV, F_POWER, F_CSD, F_COH, true, false)    \
@@ -217,7 +245,7 @@ MSIG_DECLARE_LAUNCHERS(4096)
       if (kind == F_COMPLEX) MSIG_LAUNCH_FRAMES_COMPLEX(NV, false, false);                 \
       else MSIG_KIND_SWITCH(MSIG_LAUNCH_FRAMES, NV, F_POWER, F_CSD, F_COH, false, false)   \
     }                                                                                      \
-    return hipGetLastError();                                                              \
+    return le != hipSuccess ? le : hipGetLastError();                                      \
   }                                                                                        \
   hipError_t launch_mean_batch_##NV(int kind, bool cplx, bool half, int sched,             \
                                     const KArgs* jobs, const WorkItem* queue, int32_t nwork, \

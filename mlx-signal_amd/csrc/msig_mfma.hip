@@ -281,13 +281,14 @@ static_assert(BT == MFMA_NFFT, "one bin per thread in the block reduction");
 
 hipError_t launch_mean_mfma256(int prec, const KArgs& a, int64_t rows, hipStream_t s) {
   const dim3 grid((unsigned)(rows * a.bpr)), block(BT);
+  hipError_t e;
   if (prec == PREC_F16)
-    hipLaunchKernelGGL(k_mean_mfma256<PREC_F16>, grid, block, 0, s, a);
+    e = launch_timed(&k_mean_mfma256<PREC_F16>, grid, block, s, a);
   else if (prec == PREC_BF16)
-    hipLaunchKernelGGL(k_mean_mfma256<PREC_BF16>, grid, block, 0, s, a);
+    e = launch_timed(&k_mean_mfma256<PREC_BF16>, grid, block, s, a);
   else
     return hipErrorInvalidValue;
-  return hipGetLastError();
+  return e != hipSuccess ? e : hipGetLastError();
 }
 
 }  // namespace msig
