@@ -77,6 +77,44 @@ def resolve_lengths(nperseg: int, noverlap, nfft):
     return noverlap, nperseg - noverlap, nfft
 
 
+def istft_lengths(nf: int, onesided: bool, nperseg, noverlap, nfft):
+    """(nperseg, nfft, nstep) of an inverse STFT over spectra of nf bins.
+
+    The spectra imply a segment of 2 (nf - 1) samples (one-sided) or nf
+    (two-sided), which is the default nperseg.  Without an explicit nfft the
+    transform length is that implied length, except that a one-sided
+    nperseg one longer than it (an odd segment, nf = (nperseg + 1) / 2)
+    transforms unpadded.  Overlap and nfft follow resolve_lengths."""
+    implied = 2 * (nf - 1) if onesided else nf
+    seg = implied if nperseg is None else check_nperseg(nperseg)
+    if nfft is None:
+        nfft = seg if (onesided and seg == implied + 1) else implied
+        noverlap, nstep, _ = resolve_lengths(seg, noverlap, max(nfft, seg))
+        return seg, nfft, nstep
+    noverlap, nstep, nfft = resolve_lengths(seg, noverlap, nfft)
+    return seg, nfft, nstep
+
+
+def istft_window(window, nperseg: int, scaling: str, fs: float):
+    """(float64 window values, synthesis scale) of an inverse STFT: a named
+    window is computed at nperseg, an array one must be 1-D of that length;
+    the scale undoes the forward transform's: sum w for a spectrum, sqrt(fs
+    sum w^2) for a density."""
+    if isinstance(window, (str, tuple)):
+        values = window_values(window, nperseg)
+    else:
+        values = np.asarray(to_numpy(window), dtype=np.float64)
+        if values.ndim != 1:
+            raise ValueError("window must be 1-D")
+        if values.shape[0] != nperseg:
+            raise ValueError(f"window must have length of {nperseg}")
+    s1, s2 = window_sums(values)
+    scales = {"spectrum": s1, "psd": math.sqrt(fs * s2)}
+    if scaling not in scales:
+        raise ValueError(f"Parameter {scaling=} not in ['spectrum', 'psd']!")
+    return values, scales[scaling]
+
+
 def segment_count(n: int, nperseg: int, nstep: int) -> int:
     return (n - nperseg) // nstep + 1
 
