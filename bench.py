@@ -134,6 +134,26 @@ def time_steps(torch, fn, bufs, steps, warmup, kernel_timing=False):
 
 
 TRAFFIC_PROFILE = "profiles/welch_64x2p20_traffic.json"
+VALU_PROFILE = "profiles/welch_64x2p20_valu_counters.json"
+
+
+def compute_bound(kernel_ms):
+    """The headline kernel against the bound that binds it, VALU issue: the
+    committed counter pass's VALU wave-instructions per launch, one issued per
+    CU per cycle at most, at the shader clock the same pass measured
+    (SQ_BUSY_CU_CYCLES over the dispatch time), against this run's kernel
+    time.  None if the summary is missing."""
+    p = ROOT / VALU_PROFILE
+    try:
+        c = json.loads(p.read_text())
+    except (ValueError, OSError):
+        return None
+    floor_us = c["sq_insts_valu_per_launch"] / c["cus"] / (c["shader_clock_ghz"] * 1e3)
+    return {"bound": "valu-issue", "valu_per_pair": c["valu_per_pair"],
+            "shader_clock_ghz": c["shader_clock_ghz"], "floor_us": floor_us,
+            "kernel_us": kernel_ms * 1e3, "frac": floor_us / (kernel_ms * 1e3),
+            "source": f"{VALU_PROFILE}: committed rocprofv3 counter pass of this kernel and "
+                      "workload (instruction count and clock); kernel_us is this run's"}
 
 
 def traffic_from_profile():
@@ -694,6 +714,7 @@ def main():
                      "traffic_source": f"{TRAFFIC_PROFILE}: committed rocprofv3 --pmc "
                                        "FETCH_SIZE/WRITE_SIZE passes of this kernel and "
                                        "workload (not measured in this run)"},
+        "compute_bound": compute_bound(ms_kernel),
         "secondary": secondary,
     }
     if cpu is not None:

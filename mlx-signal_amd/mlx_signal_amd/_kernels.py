@@ -409,8 +409,9 @@ def coherence_mean(x, y, win, nperseg: int, hop: int, detrend):
 
 
 def kernel_timing_arm(max_launches: int) -> None:
-    """Record HIP events around the fused transform kernel of the next
-    ``max_launches`` mean / frames calls on the current device (instrumentation)."""
+    """Time the fused transform kernel of the next ``max_launches`` mean /
+    frames calls on the current device: each launch carries a HIP event pair
+    stamped with its dispatch's start and end (instrumentation)."""
     _hip.check(_hip.load().msig_kernel_timing_arm(int(max_launches)), "kernel timing")
 
 
