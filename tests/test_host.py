@@ -269,3 +269,48 @@ def test_window_sums_cached_only_for_read_only_windows():
     mine *= 2.0
     b = _plan.window_sums(mine)
     assert b[0] == pytest.approx(2 * a[0]) and b[1] == pytest.approx(4 * a[1])
+
+
+def test_istft_lengths_match_oracle_plan():
+    """The device istft's integer plan (nperseg, nfft, nstep) equals the
+    oracle's restatement of the reference's (spectral.py:888-912) on every
+    combination, errors included (same type and text)."""
+    import re
+
+    import oracle.spectral_fp64 as osp
+
+    cases = 0
+    for nf in (1, 2, 5, 9, 65, 129):
+        for onesided in (True, False):
+            for nperseg in (None, 0, 1, 7, 8, 16, 17, 128, 257):
+                for noverlap in (None, 0, 3, 8, 200):
+                    for nfft in (None, 4, 16, 256, 300):
+                        args = (nf, onesided, nperseg, noverlap, nfft)
+                        try:
+                            want = osp.istft_plan(*args)
+                        except ValueError as e:
+                            with pytest.raises(ValueError, match=re.escape(str(e))):
+                                _plan.istft_lengths(*args)
+                            cases += 1
+                            continue
+                        assert _plan.istft_lengths(*args) == want, args
+                        cases += 1
+    assert cases == 6 * 2 * 9 * 5 * 5
+
+
+def test_istft_window_scale_and_errors():
+    import re
+
+    for win in ("hann", ("tukey", 0.25), np.hanning(64)):
+        n = 64
+        vals, s = _plan.istft_window(win, n, "spectrum", 3.0)
+        assert vals.dtype == np.float64 and vals.shape == (n,)
+        np.testing.assert_allclose(s, vals.sum(), rtol=1e-15)
+        _, sp = _plan.istft_window(win, n, "psd", 3.0)
+        np.testing.assert_allclose(sp, np.sqrt(3.0 * np.sum(vals ** 2)), rtol=1e-14)
+    with pytest.raises(ValueError, match="window must be 1-D"):
+        _plan.istft_window(np.ones((2, 8)), 8, "spectrum", 1.0)
+    with pytest.raises(ValueError, match="window must have length of 16"):
+        _plan.istft_window(np.ones(8), 16, "spectrum", 1.0)
+    with pytest.raises(ValueError, match=re.escape("Parameter scaling='x' not in ['spectrum', 'psd']!")):
+        _plan.istft_window("hann", 16, "x", 1.0)
