@@ -95,11 +95,6 @@ __device__ __forceinline__ f4 split_mma(const typename S::V& ah, const typename 
   return S::mma(ah, bh, d);
 }
 
-__device__ __forceinline__ void wave_sum2(float& a, float& b) {
-  a = lanes_sum<64>(a);
-  b = lanes_sum<64>(b);
-}
-
 __device__ __forceinline__ float wave_max(float a) {
   a = fmaxf(a, dpp_f<0xB1>(a));
   a = fmaxf(a, dpp_f<0x4E>(a));
@@ -130,7 +125,8 @@ struct MfmaLane {
     if (detrend) {
       m0 = (x[0] + x[2]) + (x[4] + x[6]);
       m1 = (x[1] + x[3]) + (x[5] + x[7]);
-      wave_sum2(m0, m1);
+      m0 = lanes_sum<64>(m0);
+      m1 = lanes_sum<64>(m1);
       m0 *= 1.f / MFMA_NFFT;
       m1 *= 1.f / MFMA_NFFT;
     }

@@ -193,9 +193,9 @@ struct UnitStream {
   // (the one it shares with the previous unit).
   MSIG_HD float first_half_sum() const {
     static_assert(HALF, "half-block sums are the half-hop stream's");
-    float s = 0.f;
+    float s = h0[0];
 #pragma unroll
-    for (int e = 0; e < H; ++e) s += h0[e];
+    for (int e = 1; e < H; ++e) s += h0[e];
     return s;
   }
 
@@ -203,12 +203,12 @@ struct UnitStream {
   // half-blocks (h1, h2).
   MSIG_HD void new_half_sums(float& pb, float& pc) const {
     static_assert(HALF, "half-block sums are the half-hop stream's");
-    pb = 0.f;
-    pc = 0.f;
+    pb = h1[0];
+    pc = h2[0];
 #pragma unroll
-    for (int e = 0; e < H; ++e) pb += h1[e];
+    for (int e = 1; e < H; ++e) pb += h1[e];
 #pragma unroll
-    for (int e = 0; e < H; ++e) pc += h2[e];
+    for (int e = 1; e < H; ++e) pc += h2[e];
   }
 
   // HALF: segment means of the fetched unit from the group sums of its
