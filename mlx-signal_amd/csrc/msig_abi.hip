@@ -608,9 +608,9 @@ constexpr int MEDR_BT = 512;
 constexpr int MEDR_CAP = 256;  // candidate keys kept per column after the first pass
 constexpr int MEDR_RANKED = 64;  // candidates ranked by counting (else more radix passes)
 
-template <int CPB, int KPT>
-__global__ __launch_bounds__(MEDR_BT) void k_median_reg(MedArgs a) {
-  constexpr int TPC = MEDR_BT / CPB;         // threads per column
+template <int CPB, int KPT, int BTM = MEDR_BT>
+__global__ __launch_bounds__(BTM) void k_median_reg(MedArgs a) {
+  constexpr int TPC = BTM / CPB;             // threads per column
   // digits of BITS bits: the first pass then leaves about a dozen keys of a
   // spectral column in the rank's bucket (some fifty at 8 bits), few enough
   // to rank by counting
@@ -622,7 +622,7 @@ __global__ __launch_bounds__(MEDR_BT) void k_median_reg(MedArgs a) {
   __shared__ uint32_t hist[CPB * HSTRIDE];
   __shared__ uint32_t cand[CPB * MEDR_CAP];
   __shared__ uint32_t ccnt[CPB];
-  __shared__ uint32_t part[MEDR_BT / 64];
+  __shared__ uint32_t part[BTM / 64];
   __shared__ uint32_t sel_bucket[CPB], sel_rank[CPB], cnt_le[CPB], min_gt[CPB], has_nan[CPB],
       kmin_s[CPB], kmax_s[CPB], done_s[CPB], sel_key[CPB];
   const int tid = threadIdx.x;
@@ -715,7 +715,7 @@ __global__ __launch_bounds__(MEDR_BT) void k_median_reg(MedArgs a) {
     const int w = hi - lo;
     const int s_lo = s_hi - BITS > 0 ? s_hi - BITS : 0;
     const int s_w = s_hi - s_lo;
-    for (int i = tid; i < CPB * HSTRIDE; i += MEDR_BT) hist[i] = 0u;
+    for (int i = tid; i < CPB * HSTRIDE; i += BTM) hist[i] = 0u;
     __syncthreads();
     if (w > 0) {
       const uint32_t dm = (1u << w) - 1u;
@@ -727,6 +727,10 @@ __global__ __launch_bounds__(MEDR_BT) void k_median_reg(MedArgs a) {
           if ((uint32_t)((uint64_t)r >> hi) == prefix)
             atomicAdd(&h[med_slot((int)((r >> lo) & dm))], 1u);
         }
+      } else if (pass == 0) {
+        // every slot is under the empty prefix (r <= range < 2^hi): no test
+#pragma unroll
+        for (int k = 0; k < KPT; ++k) atomicAdd(&h[med_slot((int)((key[k] >> lo) & dm))], 1u);
       } else {
 #pragma unroll
         for (int k = 0; k < KPT; ++k) {
@@ -875,15 +879,18 @@ __global__ __launch_bounds__(MEDR_BT) void k_median_reg(MedArgs a) {
 // (columns per block, keys per thread) of the register kernel for a column
 // length, or {0, 0} when it is longer than the register kernel holds.
 struct MedRegShape {
-  int cpb, kpt;
+  int cpb, kpt, bt;
 };
 static MedRegShape medr_shape(int nseg) {
-  if (nseg <= 16 * (MEDR_BT / 16)) return {16, 16};
-  if (nseg <= 32 * (MEDR_BT / 16)) return {16, 32};
-  if (nseg <= 32 * (MEDR_BT / 8)) return {8, 32};
-  if (nseg <= 32 * (MEDR_BT / 4)) return {4, 32};
-  if (nseg <= 64 * (MEDR_BT / 4)) return {4, 64};
-  return {0, 0};
+  if (nseg <= 16 * (MEDR_BT / 16)) return {16, 16, MEDR_BT};
+  if (nseg <= 32 * (MEDR_BT / 16)) return {16, 32, MEDR_BT};
+  // 256-thread blocks of four columns (four blocks per CU, 21 KB of LDS each)
+  // against 512-thread blocks of eight (two per CU): 121.4 -> 117.5 us on
+  // 64 x 2047 x 513.  Halving the blocks of the shorter shapes was slower.
+  if (nseg <= 32 * 64) return {4, 32, 256};
+  if (nseg <= 32 * (MEDR_BT / 4)) return {4, 32, MEDR_BT};
+  if (nseg <= 64 * (MEDR_BT / 4)) return {4, 64, MEDR_BT};
+  return {0, 0, 0};
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -1904,12 +1911,16 @@ int msig_median_frames(const float* frames, int64_t rows, int32_t nseg, int32_t 
     if (rows * (int64_t)ncb > INT32_MAX) return fail(MSIG_EINVAL, "too many columns");
     MedArgs m{frames, out, rows, nseg, cols, ncb, channels, apply_factor ? 1 : 0, nfft, 0,
               (float)scale};
-    const dim3 grid((unsigned)(rows * ncb)), block(MEDR_BT);
-    if (rs.cpb == 16 && rs.kpt == 16) hipLaunchKernelGGL((k_median_reg<16, 16>), grid, block, 0, s, m);
-    else if (rs.cpb == 16 && rs.kpt == 32) hipLaunchKernelGGL((k_median_reg<16, 32>), grid, block, 0, s, m);
-    else if (rs.cpb == 8) hipLaunchKernelGGL((k_median_reg<8, 32>), grid, block, 0, s, m);
-    else if (rs.kpt == 32) hipLaunchKernelGGL((k_median_reg<4, 32>), grid, block, 0, s, m);
-    else hipLaunchKernelGGL((k_median_reg<4, 64>), grid, block, 0, s, m);
+    const dim3 grid((unsigned)(rows * ncb)), block(rs.bt);
+#define MEDR(CP, KP, BB)                                                          \
+  else if (rs.cpb == CP && rs.kpt == KP && rs.bt == BB)                           \
+    hipLaunchKernelGGL((k_median_reg<CP, KP, BB>), grid, block, 0, s, m);
+    if (false) {
+    }
+    MEDR(16, 16, 512) MEDR(16, 32, 512) MEDR(8, 32, 512) MEDR(4, 32, 512) MEDR(4, 64, 512)
+    MEDR(8, 16, 256) MEDR(8, 32, 256) MEDR(4, 32, 256)
+    else return fail(MSIG_EINVAL, "median shape (%d, %d, %d) not built", rs.cpb, rs.kpt, rs.bt);
+#undef MEDR
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "median kernel launch");
     return 0;
