@@ -724,7 +724,46 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
         fft_passes<N, E, TH::LDS>(t, zy, tw, lds);
         separate<N, E>(t, zy, lds, y0, y1);
       }
-      if (active) {
+      // One frame's bins to the outputs: bin k of register e, onesided
+      // factor f (kinds other than complex frames).
+      auto put = [&](int64_t fr, int h, int e, int k, float f) {
+        const size_t o = (size_t)fr * nb + k;
+        const cf X = h ? x1[e] : x0[e];
+        if constexpr (OK == F_COMPLEX) {
+          static_cast<cf*>(a.out0)[o] = cf{X.re * os, X.im * os};
+        } else if constexpr (OK == F_POWER) {
+          static_cast<float*>(a.out0)[o] = (X.re * X.re + X.im * X.im) * (os * f);
+        } else {
+          const cf Y = h ? y1[e] : y0[e];
+          const cf q = cmulc(X, Y);
+          if constexpr (OK == F_CSD) {
+            static_cast<cf*>(a.out0)[o] = cf{q.re * os * f, q.im * os * f};
+          } else {
+            static_cast<float*>(a.out0)[o] = (X.re * X.re + X.im * X.im) * (os * f);
+            static_cast<float*>(a.out1)[o] = (Y.re * Y.re + Y.im * Y.im) * (os * f);
+            static_cast<cf*>(a.out2)[o] = cf{q.re * os * f, q.im * os * f};
+          }
+        }
+      };
+      if (N >= 32 && active && nb == N / 2 + 1) {
+        // onesided frames: registers e < E/2 hold bins below N/2 on every
+        // lane, and bin N/2 is register E/2 of lane 0, so the bins are
+        // stored without per-bin tests and the other registers' separation
+        // is never computed (N = 16, one thread per FFT, keeps the generic
+        // stores: both forms in one kernel cost it a wave of occupancy)
+        const int64_t s0 = 2 * (int64_t)unit;
+        const float fa = a.apply_factor ? 2.f : 1.f;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int64_t s = s0 + h;
+          if (s >= a.nseg) break;
+          const int64_t fr = row * a.nseg + s;
+          put(fr, h, 0, t, t == 0 ? 1.f : fa);
+#pragma unroll
+          for (int e = 1; e < E / 2; ++e) put(fr, h, e, t + TPF * e, fa);
+          if (t == 0) put(fr, h, E / 2, N / 2, 1.f);
+        }
+      } else if (active) {
         const int64_t s0 = 2 * (int64_t)unit;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -735,25 +774,9 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
           for (int e = 0; e < E; ++e) {
             const int k = t + TPF * e;
             if (k >= nb) continue;
-            const size_t o = (size_t)fr * nb + k;
-            const cf X = h ? x1[e] : x0[e];
             float f = 1.f;
             if (a.apply_factor) f = (k == 0 || 2 * k == N) ? 1.f : 2.f;
-            if constexpr (OK == F_COMPLEX) {
-              static_cast<cf*>(a.out0)[o] = cf{X.re * os, X.im * os};
-            } else if constexpr (OK == F_POWER) {
-              static_cast<float*>(a.out0)[o] = (X.re * X.re + X.im * X.im) * (os * f);
-            } else {
-              const cf Y = h ? y1[e] : y0[e];
-              const cf q = cmulc(X, Y);
-              if constexpr (OK == F_CSD) {
-                static_cast<cf*>(a.out0)[o] = cf{q.re * os * f, q.im * os * f};
-              } else {
-                static_cast<float*>(a.out0)[o] = (X.re * X.re + X.im * X.im) * (os * f);
-                static_cast<float*>(a.out1)[o] = (Y.re * Y.re + Y.im * Y.im) * (os * f);
-                static_cast<cf*>(a.out2)[o] = cf{q.re * os * f, q.im * os * f};
-              }
-            }
+            put(fr, h, e, k, f);
           }
         }
       }
