@@ -204,17 +204,85 @@ struct Plan {
   // LDS slot of logical position pos: one pad slot every 16 keeps the
   // stride-16 pass-0 stores and the stride-1 reads bank-conflict free.
   static MSIG_HD int pad(int pos) { return pos + (pos >> 4); }
-  // pad(a + b) for a thread part a >= 0 and a compile-time register part b,
-  // written so that it is (thread base) + (constant) whenever a or b is a
-  // multiple of 16: the compiler then keeps one LDS base per pass and folds
-  // the constants into the instructions' offset fields, instead of one
-  // address register per element.
-  static MSIG_HD int pad2(int a, int b, bool a16, bool b16) {
-    if (b16) return pad(a) + b + (b >> 4);
-    if (a16) return a + (a >> 4) + b + (b >> 4);
-    return pad(a + b);
-  }
   static constexpr int LDS = N + N / 16;
+
+  // Split exchange (one float per slot, Plan::LDS slots per FFT group): the
+  // slot of position pos at the boundary into pass P (pass P - 1's stores,
+  // pass P's loads) is pos + k (pos >> sh).  (sh, k) is chosen per boundary
+  // at compile time: among the pairs that keep every slot below LDS and keep
+  // each slot a thread part plus a register part (slot(a + b) = slot(a) +
+  // slot(b), so the register parts fold into the instructions' offset
+  // fields), the one whose stores and loads repeat the fewest LDS banks
+  // (XBANKS banks of 4 B, XLANES lanes served together); ties keep pad().
+  // 32 banks and 32 lanes per cycle is the model that the bank-conflict
+  // counter (SQ_LDS_BANK_CONFLICT) confirms for these 4-byte accesses on
+  // gfx950: welch N = 1024 reads 2^23 conflict cycles per call with pad(),
+  // 2^22 with the pads chosen for 64 banks x 64 lanes, and 0 with these.
+  static constexpr int XBANKS = 32, XLANES = 32;
+  struct XPad {
+    int sh, k;
+  };
+  static constexpr int xpad_of(XPad x, int pos) { return pos + x.k * (pos >> x.sh); }
+  // out_pos / in_pos as a thread part a(t) plus a register part b(e); false
+  // when pass p's stores have no such form (NS > TPF, final passes only)
+  static constexpr bool xsplit(int p, bool st) { return !st || TPF % ns(p) == 0; }
+  static constexpr int xa(int p, bool st, int t) {
+    return st ? (t / ns(p)) * ns(p) * radix(p) + (t % ns(p)) : t;
+  }
+  static constexpr int xb(int p, bool st, int e) {
+    const int r = radix(p), u = e / r, j = e % r;
+    return st ? u * TPF * r + j * ns(p) : TPF * u + (N / r) * j;
+  }
+  // extra bank cycles of one access set (stores of pass p or loads of p)
+  static constexpr int xcost(XPad x, int p, bool st) {
+    const int r = radix(p), nsp = ns(p), waves = TPF > 64 ? TPF / 64 : 1;
+    int cost = 0;
+    for (int e = 0; e < E; ++e)
+      for (int w = 0; w < waves; ++w)
+        for (int c = 0; c < 64; c += XLANES) {
+          int cnt[XBANKS] = {};
+          int worst = 0;
+          for (int l = c; l < c + XLANES; ++l) {
+            const int tid = w * 64 + l, g = tid / TPF, t = tid % TPF;
+            const int u = e / r, j = e % r, vt = t + TPF * u;
+            const int pos = st ? (vt / nsp) * nsp * r + (vt % nsp) + j * nsp : t + TPF * u + (N / r) * j;
+            const int b = (g * LDS + xpad_of(x, pos)) % XBANKS;
+            worst = ++cnt[b] > worst ? cnt[b] : worst;
+          }
+          cost += worst - 1;
+        }
+    return cost;
+  }
+  // pad(a + b) = pad(a) + pad(b) for every thread and register part exactly
+  // when the largest low parts (mod 2^sh) of a and of b sum below 2^sh
+  static constexpr bool xlinear(XPad x, int p, bool st) {
+    if (!xsplit(p, st)) return false;
+    const int m = (1 << x.sh) - 1;
+    int ma = 0, mb = 0;
+    for (int t = 0; t < TPF; ++t) ma = (xa(p, st, t) & m) > ma ? (xa(p, st, t) & m) : ma;
+    for (int e = 0; e < E; ++e) mb = (xb(p, st, e) & m) > mb ? (xb(p, st, e) & m) : mb;
+    return ma + mb <= m;
+  }
+  static constexpr bool xvalid(XPad x, int p) {
+    return x.k * ((N - 1) >> x.sh) + N <= LDS && (!xsplit(p - 1, true) || xlinear(x, p - 1, true)) &&
+           xlinear(x, p, false);
+  }
+  static constexpr XPad xpad_choice(int p) {
+    XPad best{4, 1};
+    int best_cost = xcost(best, p - 1, true) + xcost(best, p, false);
+    constexpr int ks[] = {1, 2, 3, 4, 5, 8};
+    for (int sh = 4; sh <= 7; ++sh)
+      for (int k : ks) {
+        const XPad x{sh, k};
+        if ((sh == 4 && k == 1) || !xvalid(x, p)) continue;
+        const int c = xcost(x, p - 1, true) + xcost(x, p, false);
+        if (c < best_cost) {
+          best = x;
+          best_cost = c;
+        }
+      }
+    return best;
+  }
   // final bin held by thread t in register e
   static MSIG_HD int bin(int t, int e) {
     constexpr int R = radix(NP - 1);
@@ -344,34 +412,37 @@ MSIG_HD void pass_compute(cf* v, const cf* tw, int t = 0) {
   }
 }
 
-// LDS slots of pass P's outputs / inputs: out_pos / in_pos split into a
-// thread part and a register part (same values, see Plan::pad2).  The split
-// exchange uses this form; the complex exchange keeps pad(pos), whose
-// per-element addresses the compiler hoists out of the loop (measured: the
-// two-signal means ran 2-4% slower with the linear form at N = 1024).
+// LDS slots of pass P's outputs / inputs.  LIN (the split exchange): the
+// boundary's own pad (Plan::xpad_choice), written as (thread part) +
+// (register part) wherever that pad allows it.  !LIN (the complex exchange)
+// keeps pad(pos), whose per-element addresses the compiler hoists out of the
+// loop (measured: the two-signal means ran 2-4% slower with the linear form
+// at N = 1024).
 template <int N, int E, int P, int S = 0, bool LIN = true>
 MSIG_HD int out_slot(int t, int e) {
   using PL = Plan<N, E, S>;
-  constexpr int R = PL::radix(P), NS = PL::ns(P), TPF = PL::TPF;
   if constexpr (!LIN) {
     return PL::pad(PL::template out_pos<P>(t, e));
-  } else if constexpr (TPF % NS == 0) {
-    const int u = e / R, j = e % R;
-    // vt = t + TPF u: vt / NS = t / NS + (TPF / NS) u, vt % NS = t % NS
-    const int a = (t / NS) * NS * R + (t % NS);
-    const int b = u * TPF * R + j * NS;
-    return PL::pad2(a, b, NS == 1 && R % 16 == 0, NS % 16 == 0 && (TPF * R) % 16 == 0);
   } else {
-    return PL::pad(PL::template out_pos<P>(t, e));
+    constexpr typename PL::XPad x = PL::xpad_choice(P + 1);
+    if constexpr (PL::xsplit(P, true) && PL::xvalid(x, P + 1))
+      return PL::xpad_of(x, PL::xa(P, true, t)) + PL::xpad_of(x, PL::xb(P, true, e));
+    else
+      return PL::xpad_of(x, PL::template out_pos<P>(t, e));
   }
 }
 template <int N, int E, int P, int S = 0, bool LIN = true>
 MSIG_HD int in_slot(int t, int e) {
   using PL = Plan<N, E, S>;
-  constexpr int R = PL::radix(P), TPF = PL::TPF;
-  if constexpr (!LIN) return PL::pad(PL::template in_pos<P>(t, e));
-  const int u = e / R, j = e % R;
-  return PL::pad2(t, TPF * u + (N / R) * j, false, TPF % 16 == 0 && (N / R) % 16 == 0);
+  if constexpr (!LIN) {
+    return PL::pad(PL::template in_pos<P>(t, e));
+  } else {
+    constexpr typename PL::XPad x = PL::xpad_choice(P);
+    if constexpr (PL::xvalid(x, P))
+      return PL::xpad_of(x, t) + PL::xpad_of(x, PL::xb(P, false, e));
+    else
+      return PL::xpad_of(x, PL::template in_pos<P>(t, e));
+  }
 }
 
 template <int N, int E, int P, int S = 0>

@@ -276,10 +276,15 @@ __device__ __forceinline__ void side_copy(const float* src, float* dst, int64_t 
 // (fft_passes SPLIT): the block's exchange buffers then take half the LDS,
 // which with the twiddle table lets four blocks share a CU.  The hop == N/2
 // welch kernels up to N = 1024 fit four waves' VGPRs without spills this way
-// (119-125); at 2048 and 4096 they need more (133, 171).
+// (119-127); at 2048 and 4096 they need more (133, 171).  The two-signal
+// kinds take it from N = 512, at their own occupancy: the split exchange's
+// linear, bank-conflict-free slots need no per-element address registers
+// (csd 236 -> 200 VGPRs at N = 1024; coherence's 12-27 spilled VGPRs at
+// 512-1024 go), where below 512 the complex exchange is already conflict
+// free and the split form only doubles the LDS instructions.
 template <int N, int KIND, bool HALF, bool CPLX = false>
 constexpr bool mean_split() {
-  return KIND == K_PSD && HALF && !CPLX && N >= 32 && N <= 1024;
+  return HALF && !CPLX && N >= (KIND == K_PSD ? 32 : 512) && N <= 1024;
 }
 template <int N, int KIND, bool HALF, bool CPLX = false>
 constexpr int mean_lds_cf() {
@@ -581,7 +586,7 @@ constexpr int mean_waves() {
     return HALF ? 2 : 1;
   }
   if constexpr (KIND == K_CSD && !HALF && !CPLX && N >= 512) return 1;
-  if constexpr (mean_split<N, KIND, HALF, CPLX>()) return 4;
+  if constexpr (KIND == K_PSD && mean_split<N, KIND, HALF, CPLX>()) return 4;
   return KIND == K_PSD ? 3 : (KIND == K_CSD ? 2 : (HALF ? 2 : 1));
 }
 // The extension stream at N = 1024 spilled at three waves per SIMD; two run
@@ -615,19 +620,24 @@ __global__ __launch_bounds__(BT, (mean_waves<N, KIND, HALF, CPLX>())) void k_mea
 
 // Per-frame spectra.  Real input: the two packed segments are separated with
 // X_s = (Z[k] + conj Z[N-k]) / 2, X_{s+1} = -i (Z[k] - conj Z[N-k]) / 2.
+// Bins go through LDS unpadded: the stores (bin(t, e), consecutive in t),
+// the reads of k and the mirrored reads of N - k all fall on distinct banks
+// (8-byte writes serve 16 lanes over 32 four-byte banks, 8-byte reads 32
+// lanes over 64), where the pos + pos/16 padding of the exchanges put two
+// lanes of every 32 on one bank pair.
 template <int N, int E>
 __device__ __forceinline__ void separate(int t, const cf* z, cf* lds, cf* x0, cf* x1) {
   using PL = Plan<N, E>;
   constexpr int TPF = PL::TPF;
   group_sync<TPF>();
 #pragma unroll
-  for (int e = 0; e < E; ++e) lds[PL::pad(PL::bin(t, e))] = z[e];
+  for (int e = 0; e < E; ++e) lds[PL::bin(t, e)] = z[e];
   group_sync<TPF>();
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = t + TPF * e;
-    const cf p = lds[PL::pad(k)];
-    const cf q = lds[PL::pad((N - k) & (N - 1))];
+    const cf p = lds[k];
+    const cf q = lds[(N - k) & (N - 1)];
     x0[e] = cf{0.5f * (p.re + q.re), 0.5f * (p.im - q.im)};
     x1[e] = cf{0.5f * (p.im + q.im), -0.5f * (p.re - q.re)};
   }
@@ -641,6 +651,10 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
   constexpr bool PAIR = (OK == F_CSD || OK == F_COH);
+  // real-input frames at 512 and 1024 exchange one component at a time
+  // (fft_passes SPLIT, conflict-free linear slots): no per-element address
+  // registers, which ends their spills (2-77 VGPRs) at the same occupancy
+  constexpr bool SPLIT = !CPLX && N >= 512 && N <= 1024;
   const int tid = threadIdx.x;
   const int g = tid / TPF, t = tid % TPF;
   cf* lds = lds_all + g * PL::LDS;
@@ -686,13 +700,13 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
       if constexpr (PAIR) sy.fetch(unit + 1, false, a);
     }
     prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
-    fft_passes<N, E, TH::LDS>(t, zx, tw, lds);
+    fft_passes<N, E, TH::LDS, 1, 0, SPLIT>(t, zx, tw, lds);
     if constexpr (CPLX) {
       // one segment per unit; bins come out in registers, no separation
       if constexpr (PAIR) {
         group_sync<TPF>();
         prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
-        fft_passes<N, E, TH::LDS>(t, zy, tw, lds);
+        fft_passes<N, E, TH::LDS, 1, 0, SPLIT>(t, zy, tw, lds);
       }
       if (active) {
         const int64_t fr = row * a.nseg + unit;
@@ -721,7 +735,7 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
       cf y0[E], y1[E];
       if constexpr (PAIR) {
         prep_unit<N, E>(zy, w, a.detrend, a.nperseg, red);
-        fft_passes<N, E, TH::LDS>(t, zy, tw, lds);
+        fft_passes<N, E, TH::LDS, 1, 0, SPLIT>(t, zy, tw, lds);
         separate<N, E>(t, zy, lds, y0, y1);
       }
       // One frame's bins to the outputs: bin k of register e, onesided
