@@ -276,15 +276,20 @@ __device__ __forceinline__ void side_copy(const float* src, float* dst, int64_t 
 // (fft_passes SPLIT): the block's exchange buffers then take half the LDS,
 // which with the twiddle table lets four blocks share a CU.  The hop == N/2
 // welch kernels up to N = 1024 fit four waves' VGPRs without spills this way
-// (119-127); at 2048 and 4096 they need more (133, 171).  The two-signal
-// kinds take it from N = 512, at their own occupancy: the split exchange's
-// linear, bank-conflict-free slots need no per-element address registers
-// (csd 236 -> 200 VGPRs at N = 1024; coherence's 12-27 spilled VGPRs at
-// 512-1024 go), where below 512 the complex exchange is already conflict
-// free and the split form only doubles the LDS instructions.
+// (119-127); at 2048 the split form keeps three waves in fewer registers
+// (168 -> 143).  The two-signal kinds take it from N = 512, at their own
+// occupancy: the split exchange's linear, bank-conflict-free slots need no
+// per-element address registers (csd 236 -> 200 VGPRs at N = 1024;
+// coherence's 12-34 spilled VGPRs at 512-4096 go), where below 512 the
+// complex exchange is already conflict free and the split form only doubles
+// the LDS instructions.  Where same-box timing found no gain (csd at 2048
+// and 4096) or a loss (welch at 4096, 106 -> 113 us at 64 x 2^20) the
+// complex exchange stays.
 template <int N, int KIND, bool HALF, bool CPLX = false>
 constexpr bool mean_split() {
-  return HALF && !CPLX && N >= (KIND == K_PSD ? 32 : 512) && N <= 1024;
+  constexpr int lo = KIND == K_PSD ? 32 : 512;
+  constexpr int hi = KIND == K_PSD ? 2048 : (KIND == K_CSD ? 1024 : 4096);
+  return HALF && !CPLX && N >= lo && N <= hi;
 }
 template <int N, int KIND, bool HALF, bool CPLX = false>
 constexpr int mean_lds_cf() {
@@ -651,10 +656,11 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
   constexpr int TPF = PL::TPF;
   constexpr int GPB = BT / TPF;
   constexpr bool PAIR = (OK == F_CSD || OK == F_COH);
-  // real-input frames at 512 and 1024 exchange one component at a time
+  // real-input frames at 512-2048 exchange one component at a time
   // (fft_passes SPLIT, conflict-free linear slots): no per-element address
-  // registers, which ends their spills (2-77 VGPRs) at the same occupancy
-  constexpr bool SPLIT = !CPLX && N >= 512 && N <= 1024;
+  // registers, which ends their spills (2-77 VGPRs at 1024) at the same
+  // occupancy (at 4096 the power frames ran 3% slower and stay complex)
+  constexpr bool SPLIT = !CPLX && N >= 512 && N <= 2048;
   const int tid = threadIdx.x;
   const int g = tid / TPF, t = tid % TPF;
   cf* lds = lds_all + g * PL::LDS;
