@@ -201,8 +201,12 @@ struct Plan {
     return o;
   }
   static constexpr int TWN = tw_off(NP) > 0 ? tw_off(NP) : 1;
-  // LDS slot of logical position pos: one pad slot every 16 keeps the
-  // stride-16 pass-0 stores and the stride-1 reads bank-conflict free.
+  // LDS slot of logical position pos in the complex (8-byte) exchange: one
+  // pad slot every 16 keeps the stride-16 stores conflict-free (8-byte
+  // writes serve 16 lanes over 32 banks); the stride-1 reads of 32 lanes
+  // meet on one bank pair once per 32 lanes (8-byte reads serve 32 lanes
+  // over 64 banks), which no pad of this additive form avoids while keeping
+  // the stores conflict-free.  The split exchange has its own pads (XPad).
   static MSIG_HD int pad(int pos) { return pos + (pos >> 4); }
   static constexpr int LDS = N + N / 16;
 

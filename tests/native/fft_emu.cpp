@@ -536,6 +536,27 @@ static int check_shared_table(std::mt19937_64& rng) {
   return ok ? 0 : 1;
 }
 
+// The split exchange's per-boundary pads (Plan::xpad_choice): every slot
+// below Plan::LDS, a thread part plus a register part, and no repeated bank
+// among any 32 lanes of a store or load under the 32-bank, 32-lane service
+// of 4-byte LDS accesses (the model the bank-conflict counter confirms).
+template <int N, int S = 0>
+static int check_split_pads() {
+  using PL = msig::Plan<N, 16, S>;
+  int bad = 0;
+  for (int p = 1; p < PL::NP; ++p) {
+    const auto x = PL::xpad_choice(p);
+    const int cost = PL::xcost(x, p - 1, true) + PL::xcost(x, p, false);
+    if (!PL::xvalid(x, p) || cost != 0) {
+      std::printf("N=%d S=%d boundary %d: pad (%d, %d) valid %d, %d conflict cycles\n", N, S, p, x.sh,
+                  x.k, (int)PL::xvalid(x, p), cost);
+      bad = 1;
+    }
+  }
+  if (!bad) std::printf("N=%d S=%d split pads conflict-free ok\n", N, S);
+  return bad;
+}
+
 int main() {
   std::mt19937_64 rng(1234);
   int bad = 0;
@@ -589,5 +610,17 @@ int main() {
   bad += check_schedule<512>(rng);
   bad += check_schedule<1024>(rng);
   bad += check_schedule<2048>(rng);
+  // split-exchange pads at every size and schedule the split kernels use
+  bad += check_split_pads<32>();
+  bad += check_split_pads<64>();
+  bad += check_split_pads<128>();
+  bad += check_split_pads<256>();
+  bad += check_split_pads<512>();
+  bad += check_split_pads<1024>();
+  bad += check_split_pads<2048>();
+  bad += check_split_pads<4096>();
+  bad += check_split_pads<512, 1>();
+  bad += check_split_pads<1024, 1>();
+  bad += check_split_pads<2048, 1>();
   return bad ? 1 : 0;
 }
