@@ -1,7 +1,8 @@
 """CPU checks of the device FFT: the host emulator (tests/native/fft_emu.cpp)
 runs the kernels' own msig_fft.hpp templates lane by lane with an emulated
-LDS and compares against an O(N^2) FP64 DFT, checks LDS slot coverage, and
-the two-for-one fold identities the kernels rely on."""
+LDS and compares against an O(N^2) FP64 DFT, checks LDS slot coverage, that
+the split exchange's pads repeat no LDS bank among 32 lanes, and the
+two-for-one fold identities the kernels rely on."""
 
 import shutil
 import subprocess
