@@ -624,7 +624,10 @@ __global__ __launch_bounds__(BT, (mean_waves<N, KIND, HALF, CPLX>())) void k_mea
 }
 
 // Per-frame spectra.  Real input: the two packed segments are separated with
-// X_s = (Z[k] + conj Z[N-k]) / 2, X_{s+1} = -i (Z[k] - conj Z[N-k]) / 2.
+// X_s = (Z[k] + conj Z[N-k]) / 2, X_{s+1} = -i (Z[k] - conj Z[N-k]) / 2;
+// separate() returns 2 X_s and 2 X_{s+1}, and the callers fold the halving
+// into their output scale (a power-of-two factor, so every output keeps its
+// bits and the four multiplies per bin go).
 // Bins go through LDS unpadded: the stores (bin(t, e), consecutive in t),
 // the reads of k and the mirrored reads of N - k all fall on distinct banks
 // (8-byte writes serve 16 lanes over 32 four-byte banks, 8-byte reads 32
@@ -643,8 +646,8 @@ __device__ __forceinline__ void separate(int t, const cf* z, cf* lds, cf* x0, cf
     const int k = t + TPF * e;
     const cf p = lds[k];
     const cf q = lds[(N - k) & (N - 1)];
-    x0[e] = cf{0.5f * (p.re + q.re), 0.5f * (p.im - q.im)};
-    x1[e] = cf{0.5f * (p.im + q.im), -0.5f * (p.re - q.re)};
+    x0[e] = cf{p.re + q.re, p.im - q.im};
+    x1[e] = cf{p.im + q.im, q.re - p.re};
   }
   group_sync<TPF>();
 }
@@ -736,6 +739,9 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
         }
       }
     } else {
+      // 2 X per separated bin: complex outputs scale by os / 2, products
+      // (powers, cross spectra) by os / 4
+      const float osx = OK == F_COMPLEX ? 0.5f * os : 0.25f * os;
       cf x0[E], x1[E];
       separate<N, E>(t, zx, lds, x0, x1);
       cf y0[E], y1[E];
@@ -750,18 +756,18 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
         const size_t o = (size_t)fr * nb + k;
         const cf X = h ? x1[e] : x0[e];
         if constexpr (OK == F_COMPLEX) {
-          static_cast<cf*>(a.out0)[o] = cf{X.re * os, X.im * os};
+          static_cast<cf*>(a.out0)[o] = cf{X.re * osx, X.im * osx};
         } else if constexpr (OK == F_POWER) {
-          static_cast<float*>(a.out0)[o] = (X.re * X.re + X.im * X.im) * (os * f);
+          static_cast<float*>(a.out0)[o] = (X.re * X.re + X.im * X.im) * (osx * f);
         } else {
           const cf Y = h ? y1[e] : y0[e];
           const cf q = cmulc(X, Y);
           if constexpr (OK == F_CSD) {
-            static_cast<cf*>(a.out0)[o] = cf{q.re * os * f, q.im * os * f};
+            static_cast<cf*>(a.out0)[o] = cf{q.re * osx * f, q.im * osx * f};
           } else {
-            static_cast<float*>(a.out0)[o] = (X.re * X.re + X.im * X.im) * (os * f);
-            static_cast<float*>(a.out1)[o] = (Y.re * Y.re + Y.im * Y.im) * (os * f);
-            static_cast<cf*>(a.out2)[o] = cf{q.re * os * f, q.im * os * f};
+            static_cast<float*>(a.out0)[o] = (X.re * X.re + X.im * X.im) * (osx * f);
+            static_cast<float*>(a.out1)[o] = (Y.re * Y.re + Y.im * Y.im) * (osx * f);
+            static_cast<cf*>(a.out2)[o] = cf{q.re * osx * f, q.im * osx * f};
           }
         }
       };
