@@ -297,6 +297,11 @@ constexpr int mean_lds_cf() {
   return (BT / PL::TPF) * (mean_split<N, KIND, HALF, CPLX>() ? PL::LDS / 2 : PL::LDS);
 }
 
+template <bool B>
+struct RevTag {
+  static constexpr bool value = B;
+};
+
 struct NoPrologue {
   __device__ void operator()() const {}
 };
@@ -434,22 +439,55 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
     // group_sum2 and must keep the test, so that their barrier sequence
     // matches prep_unit's.
     constexpr bool DALL = TPF <= 64 && KIND == K_PSD;
-    auto means = [&](const UnitStream<N, E, CPLX, HALF>& st, float& carry, float& mr, float& mi) {
+    auto means = [&](auto rev, const UnitStream<N, E, CPLX, HALF>& st, float& carry, float& mr,
+                     float& mi) {
+      constexpr bool REV = decltype(rev)::value;
       mr = 0.f;
       mi = 0.f;
       if (DALL || a.detrend) {
         float sb, sc;
-        st.new_half_sums(sb, sc);
+        st.template new_half_sums<REV>(sb, sc);
         group_sum2<TPF>(sb, sc, red);
         st.half_means(carry, sb, sc, dinv, mr, mi);
       }
     };
-    if (has_full) {
+    // welch from N = 1024: each unit's prefetch goes into the half-blocks the
+    // unit no longer needs (UnitStream::fetch_next), so the loop runs two
+    // units per trip with the roles swapped in the second and copies no
+    // half-block (617 -> 602 VALU per pair in the standalone kernel's loop,
+    // 127 -> 123 VGPRs; 64 x 2^20: 86.7 -> 85.7 us per call).  The loads then
+    // issue after the unit is taken instead of ahead of it, which the short
+    // transforms of small N do not cover (N = 256: 67.3 -> 69.5 us), so
+    // those keep the copy.  Every welch body of one N (standalone, batch,
+    // single-launch batch) runs the same loop, so a batched call keeps the
+    // standalone call's bits.  The two-signal kinds keep one unit per trip
+    // and the copied carry (fetch_full): unrolled, csd and coherence spilled
+    // 54 and 82 VGPRs.
+    constexpr bool ROTATE = KIND == K_PSD && N >= 1024;
+    if constexpr (ROTATE) {
+      auto full_unit = [&](auto rev, int nxt) {
+        constexpr bool REV = decltype(rev)::value;
+        cf zx[E], zy[E];
+        float mrx, mix;
+        means(rev, sx, sxc, mrx, mix);
+        sx.template take_full_prepped<REV>(zx, w, DALL || a.detrend, mrx, mix);
+        sx.template fetch_next<REV>(nxt);
+        transform_acc(zx, zy, true, [] {});
+      };
+      if (has_full) {
+        int unit = unit0;
+        for (; unit + 1 < uf; unit += 2) {
+          full_unit(RevTag<false>{}, unit + 1);
+          full_unit(RevTag<true>{}, unit + 2 < uf ? unit + 2 : uf - 1);
+        }
+        if (unit < uf) full_unit(RevTag<false>{}, uf - 1);
+      }
+    } else if (has_full) {
       for (int unit = unit0; unit < uf; ++unit) {
         const int nxt = unit + 1 < uf ? unit + 1 : uf - 1;
         cf zx[E], zy[E];
         float mrx, mix;
-        means(sx, sxc, mrx, mix);
+        means(RevTag<false>{}, sx, sxc, mrx, mix);
         sx.take_full_prepped(zx, w, DALL || a.detrend, mrx, mix);
         sx.fetch_full(nxt, false);
         if constexpr (KIND == K_PSD) {
@@ -460,7 +498,7 @@ __device__ __forceinline__ void mean_body(const KArgs& a, int64_t row, int bir, 
           // lets hop == N/2 coherence run two waves per SIMD)
           transform_acc(zx, zy, true, [&] {
             float mry, miy;
-            means(sy, syc, mry, miy);
+            means(RevTag<false>{}, sy, syc, mry, miy);
             sy.take_full_prepped(zy, w, DALL || a.detrend, mry, miy);
             sy.fetch_full(nxt, false);
           });

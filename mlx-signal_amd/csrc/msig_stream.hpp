@@ -189,6 +189,34 @@ struct UnitStream {
     pair_ok = true;
   }
 
+  // HALF, the rotating form of the full-unit loop.  A unit's three
+  // half-blocks are (first, h1, last) with first = h0 and last = h2 when REV
+  // is false and the two swapped when it is true.  fetch_next<REV> loads the
+  // next unit into h1 and into `first`, whose half is dead once the unit is
+  // taken, so the next unit is (last, h1, first): REV alternates from unit
+  // to unit and the shared half-block stays in its registers instead of
+  // being copied (a loop unrolled by two names every array statically).
+  template <bool REV>
+  MSIG_HD float* first_half() {
+    return REV ? h2 : h0;
+  }
+  template <bool REV>
+  MSIG_HD const float* first_half() const {
+    return REV ? h2 : h0;
+  }
+  template <bool REV>
+  MSIG_HD const float* last_half() const {
+    return REV ? h0 : h2;
+  }
+  template <bool REV>
+  MSIG_HD void fetch_next(int unit) {
+    static_assert(HALF, "full-unit fetch is the half-hop stream's");
+    const int64_t s0 = 2 * (int64_t)unit;
+    half_load(h1, s0 + 1, true);
+    half_load(first_half<REV>(), s0 + 2, true);
+    pair_ok = true;
+  }
+
   // HALF: this thread's partial sum of the fetched unit's first half-block
   // (the one it shares with the previous unit).
   MSIG_HD float first_half_sum() const {
@@ -200,15 +228,17 @@ struct UnitStream {
   }
 
   // HALF: this thread's partial sums of the fetched unit's two new
-  // half-blocks (h1, h2).
+  // half-blocks (h1 and the last one).
+  template <bool REV = false>
   MSIG_HD void new_half_sums(float& pb, float& pc) const {
     static_assert(HALF, "half-block sums are the half-hop stream's");
+    const float* hl = last_half<REV>();
     pb = h1[0];
-    pc = h2[0];
+    pc = hl[0];
 #pragma unroll
     for (int e = 1; e < H; ++e) pb += h1[e];
 #pragma unroll
-    for (int e = 1; e < H; ++e) pc += h2[e];
+    for (int e = 1; e < H; ++e) pc += hl[e];
   }
 
   // HALF: segment means of the fetched unit from the group sums of its
@@ -225,19 +255,22 @@ struct UnitStream {
   // HALF: the points of a unit fetched with fetch_full, detrended by the
   // segment means (mr, mi) when `detrend` and windowed by w[e] (sample
   // t + TPF e).
+  template <bool REV = false>
   MSIG_HD void take_full_prepped(cf* z, const float* w, bool detrend, float mr, float mi) const {
     static_assert(HALF, "full-unit take is the half-hop stream's");
+    const float* hf = first_half<REV>();
+    const float* hl = last_half<REV>();
     if (detrend) {
 #pragma unroll
       for (int e = 0; e < H; ++e) {
-        z[e] = cf{(h0[e] - mr) * w[e], (h1[e] - mi) * w[e]};
-        z[e + H] = cf{(h1[e] - mr) * w[e + H], (h2[e] - mi) * w[e + H]};
+        z[e] = cf{(hf[e] - mr) * w[e], (h1[e] - mi) * w[e]};
+        z[e + H] = cf{(h1[e] - mr) * w[e + H], (hl[e] - mi) * w[e + H]};
       }
     } else {
 #pragma unroll
       for (int e = 0; e < H; ++e) {
-        z[e] = cf{h0[e] * w[e], h1[e] * w[e]};
-        z[e + H] = cf{h1[e] * w[e + H], h2[e] * w[e + H]};
+        z[e] = cf{hf[e] * w[e], h1[e] * w[e]};
+        z[e + H] = cf{h1[e] * w[e + H], hl[e] * w[e + H]};
       }
     }
   }

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "msig_fft.hpp"
@@ -320,12 +321,15 @@ static int check_stream(std::mt19937_64& rng, int nperseg, int hop, int n, int d
 // the next fetch clamped to the group's last full unit, and take their points
 // with take_full_prepped, detrended by means built from half-block group
 // sums (first_half_sum once per group, new_half_sums per unit, half_means
-// carrying the shared half's sum), as the kernel does.  The partner-less unit
-// of an odd segment count runs once through the general fetch.  Checks
-// coverage (each unit once), that every fetch_full stays inside the row, and
-// the mean against the FP64 DFT.
+// carrying the shared half's sum), as the kernel does.  rotate: the
+// standalone welch kernel's loop instead, two units per trip with the
+// half-block roles swapped in the second (fetch_next, REV alternating) and
+// an odd last unit alone.  The partner-less unit of an odd segment count
+// runs once through the general fetch.  Checks coverage (each unit once),
+// that every fetch stays inside the row, and the mean against the FP64 DFT.
 template <int N>
-static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detrend) {
+static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detrend,
+                             bool rotate = false) {
   constexpr int E = N < 16 ? N : 16;
   constexpr int TPF = N / E;
   const int hop = N / 2;
@@ -391,14 +395,14 @@ static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detren
       float carry = 0.f;  // group sum of the shared half-block
       if (detrend)
         for (int t = 0; t < TPF; ++t) carry += st[t].first_half_sum();
-      for (int unit = unit0; unit < uf; ++unit) {
-        const int nxt = unit + 1 < uf ? unit + 1 : uf - 1;
+      auto one_unit = [&](auto rev, int unit, int nxt) -> bool {
+        constexpr bool REV = decltype(rev)::value;
         float mr = 0.f, mi = 0.f;
         if (detrend) {
           float sb = 0.f, sc = 0.f;
           for (int t = 0; t < TPF; ++t) {
             float b, c;
-            st[t].new_half_sums(b, c);
+            st[t].template new_half_sums<REV>(b, c);
             sb += b;
             sc += c;
           }
@@ -407,12 +411,31 @@ static int check_half_groups(std::mt19937_64& rng, int nseg, int run, int detren
         std::vector<cf> z(N);
         for (int t = 0; t < TPF; ++t) {
           cf zt[E];
-          st[t].take_full_prepped(zt, wt[t].data(), detrend != 0, mr, mi);
+          st[t].template take_full_prepped<REV>(zt, wt[t].data(), detrend != 0, mr, mi);
           for (int e = 0; e < E; ++e) z[t + TPF * e] = zt[e];
         }
-        full_fetch(nxt, false);
-        if (!accumulate(z)) return 1;
+        if (rotate) {
+          in_bounds = in_bounds && (int64_t)(2 * nxt + 3) * hop <= n;
+          for (int t = 0; t < TPF; ++t) st[t].template fetch_next<REV>(nxt);
+        } else {
+          full_fetch(nxt, false);
+        }
+        if (!accumulate(z)) return false;
         ++seen[unit];
+        return true;
+      };
+      using F = std::integral_constant<bool, false>;
+      using T = std::integral_constant<bool, true>;
+      int unit = unit0;
+      if (rotate) {
+        for (; unit + 1 < uf; unit += 2) {
+          if (!one_unit(F{}, unit, unit + 1)) return 1;
+          if (!one_unit(T{}, unit + 1, unit + 2 < uf ? unit + 2 : uf - 1)) return 1;
+        }
+        if (unit < uf && !one_unit(F{}, unit, uf - 1)) return 1;
+      } else {
+        for (; unit < uf; ++unit)
+          if (!one_unit(F{}, unit, unit + 1 < uf ? unit + 1 : uf - 1)) return 1;
       }
     }
     if (full >= unit0 && full < u1) {
@@ -603,6 +626,15 @@ int main() {
   bad += check_half_groups<256>(rng, 9, 1, 1);
   bad += check_half_groups<1024>(rng, 13, 2, 1);
   bad += check_half_groups<1024>(rng, 1, 4, 1);
+  // the same with the standalone welch kernel's rotating two-unit loop
+  bad += check_half_groups<64>(rng, 15, 3, 1, true);
+  bad += check_half_groups<64>(rng, 16, 3, 0, true);
+  bad += check_half_groups<256>(rng, 21, 4, 1, true);
+  bad += check_half_groups<256>(rng, 9, 1, 1, true);
+  bad += check_half_groups<1024>(rng, 13, 2, 1, true);
+  bad += check_half_groups<1024>(rng, 1, 4, 1, true);
+  bad += check_half_groups<1024>(rng, 33, 16, 1, true);
+  bad += check_half_groups<256>(rng, 40, 7, 0, true);
   // radix schedule 1 wherever it differs from schedule 0
   bad += check_schedule<32>(rng);
   bad += check_schedule<64>(rng);

@@ -23,4 +23,4 @@ def test_fft_emulator(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count(" ok") == 54
+    assert r.stdout.count(" ok") == 62
