@@ -257,8 +257,9 @@ struct Plan {
         }
     return cost;
   }
-  // pad(a + b) = pad(a) + pad(b) for every thread and register part exactly
-  // when the largest low parts (mod 2^sh) of a and of b sum below 2^sh
+  // slot(a + b) = slot(a) + slot(b) for every thread part a and register
+  // part b when the largest low parts (mod 2^sh) of a and of b sum below 2^sh
+  // (no carry into bit sh, so (a + b) >> sh = (a >> sh) + (b >> sh))
   static constexpr bool xlinear(XPad x, int p, bool st) {
     if (!xsplit(p, st)) return false;
     const int m = (1 << x.sh) - 1;
