@@ -138,22 +138,31 @@ VALU_PROFILE = "profiles/welch_64x2p20_valu_counters.json"
 
 
 def compute_bound(kernel_ms):
-    """The headline kernel against the bound that binds it, VALU issue: the
-    committed counter pass's VALU wave-instructions per launch, one issued per
-    CU per cycle at most, at the shader clock the same pass measured
-    (SQ_BUSY_CU_CYCLES over the dispatch time), against this run's kernel
-    time.  None if the summary is missing."""
+    """The headline kernel against the bound that binds it, VALU issue.  The
+    committed counter pass gives the VALU wave-instructions per launch (a
+    property of the build, the same on every box) and the shader clock of the
+    box it ran on; a CU issues at most one per cycle.  The shader clock this
+    load sustains differs between boxes (1.97-2.15 GHz measured), and a run
+    cannot read it without counters, so the line reports the clock at which
+    this run's kernel time equals 100% issue (issue_clock_ghz) beside the
+    counter pass's own clock and floor.  None if the summary is missing."""
     p = ROOT / VALU_PROFILE
     try:
         c = json.loads(p.read_text())
     except (ValueError, OSError):
         return None
-    floor_us = c["sq_insts_valu_per_launch"] / c["cus"] / (c["shader_clock_ghz"] * 1e3)
+    valu_per_cu = c["sq_insts_valu_per_launch"] / c["cus"]
+    floor_us = valu_per_cu / (c["shader_clock_ghz"] * 1e3)
+    kernel_us = kernel_ms * 1e3
     return {"bound": "valu-issue", "valu_per_pair": c["valu_per_pair"],
-            "shader_clock_ghz": c["shader_clock_ghz"], "floor_us": floor_us,
-            "kernel_us": kernel_ms * 1e3, "frac": floor_us / (kernel_ms * 1e3),
+            "valu_per_cu": valu_per_cu, "kernel_us": kernel_us,
+            "issue_clock_ghz": valu_per_cu / (kernel_us * 1e3),
+            "counter_pass": {"shader_clock_ghz": c["shader_clock_ghz"],
+                             "valu_issue_per_cu_cycle": c["valu_issue_per_cu_cycle"],
+                             "floor_us": floor_us, "dispatch_us": c["dispatch_us"]},
             "source": f"{VALU_PROFILE}: committed rocprofv3 counter pass of this kernel and "
-                      "workload (instruction count and clock); kernel_us is this run's"}
+                      "workload (instruction count, and the clock of the box it ran on); "
+                      "kernel_us is this run's"}
 
 
 def traffic_from_profile():
