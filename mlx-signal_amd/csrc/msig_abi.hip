@@ -881,9 +881,14 @@ __global__ __launch_bounds__(BTM) void k_median_reg(MedArgs a) {
 struct MedRegShape {
   int cpb, kpt, bt;
 };
-static MedRegShape medr_shape(int nseg) {
+static MedRegShape medr_shape(int nseg, int64_t columns) {
   if (nseg <= 16 * (MEDR_BT / 16)) return {16, 16, MEDR_BT};
-  if (nseg <= 32 * (MEDR_BT / 16)) return {16, 32, MEDR_BT};
+  // 513-1024 segments: 256-thread blocks of eight columns (four blocks per
+  // CU) once the call has a few rounds of them: 128 x 1023 x 513 ran 187 ->
+  // 132 us against 512-thread blocks of sixteen; small calls (16 x 1023 x
+  // 129: 17.5 against 18.7 us) and 512 or fewer segments (256 x 511 x 513:
+  // 151 against 186 us) keep the wider blocks.
+  if (nseg <= 32 * (MEDR_BT / 16)) return columns >= 8192 ? MedRegShape{8, 32, 256} : MedRegShape{16, 32, MEDR_BT};
   // 256-thread blocks of four columns (four blocks per CU, 21 KB of LDS each)
   // against 512-thread blocks of eight (two per CU): 121.4 -> 117.5 us on
   // 64 x 2047 x 513.  Halving the blocks of the shorter shapes was slower.
@@ -1905,7 +1910,7 @@ int msig_median_frames(const float* frames, int64_t rows, int32_t nseg, int32_t 
   if (nseg > MSIG_MEDIAN_MAX_NSEG)
     return fail(MSIG_ENOTSUP, "nseg %d above the staged median's %d", nseg, MSIG_MEDIAN_MAX_NSEG);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const MedRegShape rs = medr_shape(nseg);
+  const MedRegShape rs = medr_shape(nseg, rows * (int64_t)cols);
   if (rs.cpb) {  // keys in registers
     const int ncb = (cols + rs.cpb - 1) / rs.cpb;
     if (rows * (int64_t)ncb > INT32_MAX) return fail(MSIG_EINVAL, "too many columns");

@@ -101,3 +101,24 @@ def test_median_kernel_adversarial_columns(rng, nseg):
     a[:, :, 5] = 7.0                                                 # constant
     got = _kernels.median_frames(torch.from_numpy(a).cuda(), 1.0, False, 0).cpu().numpy()
     np.testing.assert_array_equal(got, np.median(a, axis=1).astype(np.float32))
+
+
+@pytest.mark.parametrize("nseg", [513, 700, 1023, 1024])
+def test_median_kernel_many_columns_mid_length(rng, nseg):
+    """513-1024 segments over at least 8192 columns: the call plans 256-thread
+    blocks of eight columns instead of 512-thread blocks of sixteen.  Exact
+    against numpy.median, including ties, signed zeros and extreme exponents,
+    and equal to the same columns run as a small call (the other shape)."""
+    rows, bins = 4, 2100
+    a = rng.exponential(size=(rows, nseg, bins)).astype(np.float32)
+    a[:, :, 5] = np.round(a[:, :, 5])
+    a[:, :, 6] = 0.0
+    a[:, ::2, 7] = -0.0
+    a[:, :, 8] = a[:, :, 8] * 1e30
+    a[:, :, 9] = a[:, :, 9] * 1e-30
+    a[:, 0, 10] = 1e30
+    got = _kernels.median_frames(torch.from_numpy(a).cuda(), 1.0, False, 0).cpu().numpy()
+    np.testing.assert_array_equal(got, np.median(a, axis=1).astype(np.float32))
+    small = _kernels.median_frames(torch.from_numpy(np.ascontiguousarray(a[:1, :, :64])).cuda(), 1.0,
+                                   False, 0).cpu().numpy()
+    np.testing.assert_array_equal(small, got[:1, :64])
