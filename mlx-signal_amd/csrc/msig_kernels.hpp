@@ -733,6 +733,71 @@ __device__ __forceinline__ void frames_body(const KArgs& a, int64_t row, int bir
   const int nb = a.nbins;
   const float os = a.oscale;
 
+  // Power frames of real input at hop == N/2 (spectrogram psd, the median
+  // average), groups of one wave: the mean kernels' full-unit stream.  The
+  // units whose two segments both exist take their points without bounds
+  // selects and detrend with the carried half-block sums (UnitStream
+  // fetch_full / take_full_prepped / half_means; means of zero without
+  // detrend, so the loop is one basic block); the partner-less unit of an
+  // odd segment count runs once through the general fetch after them.
+  constexpr bool FULL = HALF && !EXT && !CPLX && OK == F_POWER && TPF <= 64 && N >= 32;
+  if constexpr (FULL) {
+   if (nb == N / 2 + 1) {  // onesided
+    const int u1 = unit0 + a.run < a.units ? unit0 + a.run : a.units;
+    const int full = (int)(a.nseg / 2);
+    const int uf = u1 < full ? u1 : full;
+    const float osx = 0.25f * os;  // separate() returns 2 X
+    const float fa = a.apply_factor ? 2.f : 1.f;
+    // |X|^2 of segment s (bins of x = x0 or x1) to frame row * nseg + s:
+    // registers e < E/2 hold bins below N/2 on every lane, bin N/2 is
+    // register E/2 of lane 0
+    auto store = [&](int64_t s, const cf (&x)[E]) {
+      const int64_t fr = row * a.nseg + s;
+      auto put = [&](int e, int k, float f) {
+        static_cast<float*>(a.out0)[(size_t)fr * nb + k] =
+            (x[e].re * x[e].re + x[e].im * x[e].im) * (osx * f);
+      };
+      put(0, t, t == 0 ? 1.f : fa);
+#pragma unroll
+      for (int e = 1; e < E / 2; ++e) put(e, t + TPF * e, fa);
+      if (t == 0) put(E / 2, N / 2, 1.f);
+    };
+    if (unit0 < uf) {
+      const float dinv = a.detrend ? 1.0f / (float)a.nperseg : 0.f;  // zero means without detrend
+      sx.fetch_full(unit0, true);
+      float carry = sx.first_half_sum(), unused = 0.f;
+      group_sum2<TPF>(carry, unused, red);
+      for (int unit = unit0; unit < uf; ++unit) {
+        const int nxt = unit + 1 < uf ? unit + 1 : uf - 1;
+        float sb, sc, mr, mi;
+        sx.new_half_sums(sb, sc);
+        group_sum2<TPF>(sb, sc, red);
+        sx.half_means(carry, sb, sc, dinv, mr, mi);
+        cf zx[E];
+        sx.take_full_prepped(zx, w, true, mr, mi);
+        sx.fetch_full(nxt, false);
+        fft_passes<N, E, TH::LDS, 1, 0, SPLIT>(t, zx, tw, lds);
+        cf x0[E], x1[E];
+        separate<N, E>(t, zx, lds, x0, x1);
+        store(2 * (int64_t)unit, x0);
+        store(2 * (int64_t)unit + 1, x1);
+      }
+    }
+    if (full >= unit0 && full < u1) {
+      cf zx[E];
+      sx.fetch(full, true, a);
+      sx.take(zx);
+      prep_unit<N, E>(zx, w, a.detrend, a.nperseg, red);
+      fft_passes<N, E, TH::LDS, 1, 0, SPLIT>(t, zx, tw, lds);
+      cf x0[E], x1[E];
+      separate<N, E>(t, zx, lds, x0, x1);
+      store(2 * (int64_t)full, x0);
+      if (2 * (int64_t)full + 1 < a.nseg) store(2 * (int64_t)full + 1, x1);
+    }
+    return;
+   }
+  }
+
   sx.fetch(unit0, true, a);
   if constexpr (PAIR) sy.fetch(unit0, true, a);
   for (int it = 0; it < a.run; ++it) {
