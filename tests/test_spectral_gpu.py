@@ -274,6 +274,30 @@ def test_spectrogram_fused_power_path(rng):
     assert_peak_close(s, oracle.spectrogram(x, nperseg=1024, noverlap=512, window="hann")[2], TOL)
 
 
+@pytest.mark.parametrize("nperseg", [32, 256, 1024])
+@pytest.mark.parametrize("odd", [False, True])
+@pytest.mark.parametrize("detrend", [False, "constant"])
+def test_spectrogram_half_hop_power_stream(rng, nperseg, odd, detrend):
+    """Power frames at hop = N/2 run the full-unit stream (no bounds selects,
+    carried half-block detrend sums) for every unit with both segments and
+    the general fetch for the partner-less last unit of an odd count: odd
+    and even segment counts, with and without detrend, on a signal with a
+    large DC offset, several rows and many units per group; two-sided output
+    keeps the general loop."""
+    hop = nperseg // 2
+    nseg = 37 * 16 + (1 if odd else 0)
+    n = nperseg + hop * (nseg - 1) + hop // 3
+    x = make_sig(rng, (3, n)) + 5.0
+    kw = dict(nperseg=nperseg, noverlap=hop, window="hann", detrend=detrend)
+    before = _kernels.launches["frames_power"]
+    _, _, s = msig.spectrogram(x, **kw)
+    assert _kernels.launches["frames_power"] == before + 1
+    assert s.shape[-1] == nseg
+    assert_peak_close(s, oracle.spectrogram(x, **kw)[2], TOL)
+    kw2 = dict(kw, return_onesided=False, scaling="spectrum")
+    assert_peak_close(msig.spectrogram(x, **kw2)[2], oracle.spectrogram(x, **kw2)[2], TOL)
+
+
 def test_spectrogram_custom_params(rng):
     x = make_sig(rng, (4096,))
     kw = dict(nperseg=200, noverlap=50, nfft=256, scaling="spectrum", detrend="linear")
